@@ -1,0 +1,824 @@
+// C ABI of the dynemol_hip library: contexts, systems and the drop-in
+// propagationelhl_gpucaller_ replacement.  Host orchestration only; every
+// arithmetic step runs in the kernels of overlap.hip, dense.hip and cheb.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dynemol_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace dh {
+static thread_local char g_err[512] = "";
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+const char* last_error() { return g_err; }
+}  // namespace dh
+
+using namespace dh;
+
+static const double H_BAR = 6.58264e-4;  // eV*ps, constants_m.f:23
+static const int KMAX_ALLOC = 4096;      // Chebyshev terms supported (r*dt up to ~3500)
+static const int LANCZOS_MAX = 120;
+
+struct dh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // shared workspace for operator builds (one build at a time per context)
+    int ws_ld = 0;
+    double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr, *trwork = nullptr;
+    int* d_info = nullptr;
+    const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
+    dh_system* dropin = nullptr;
+    int dropin_N = 0;
+};
+
+struct dh_system {
+    dh_context* ctx = nullptr;
+    int n_atoms = 0, N = 0, ld = 0, W = 0, n_frag = 1;
+    // basis
+    BasisDev bd{};
+    void* basis_mem = nullptr;
+    double* coords = nullptr;
+    int* frag = nullptr;
+    // operators
+    double *Q1 = nullptr, *Q2 = nullptr, *Hth = nullptr;
+    // propagation buffers
+    cplx *phi = nullptr, *x0 = nullptr, *x1 = nullptr, *acc = nullptr, *out = nullptr, *hk = nullptr;
+    cplx* partial = nullptr;
+    unsigned* counters = nullptr;
+    int split_max = 1;
+    // Lanczos / Bessel
+    cplx *lv = nullptr, *lvp = nullptr, *lw = nullptr;
+    double *lal = nullptr, *lbe = nullptr, *bounds = nullptr, *J = nullptr;
+    cplx* coef = nullptr;
+    int* kout = nullptr;
+    // observables
+    double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr;
+    // host state
+    bool have_coords = false, ops_ready = false, phi_dirty = true, state_set = false, hprime_ready = false;
+    double bounds_h[6] = {0, 0, 0, 0, 0, 0};
+    double emin = 0, emax = 0;
+    int lanczos_steps = 48;
+    double rel_margin = 0.02;
+    double widen = 1.0;
+    double coef_dt = -1.0, coef_c = 0.0, coef_r = 0.0;
+    int K = 0;
+    double stats[5] = {0, 0, 0, 0, 0};
+};
+
+#define CHECK_ARG(cond, msg)                  \
+    do {                                      \
+        if (!(cond)) {                        \
+            dh::set_error("%s", msg);         \
+            return DH_ERR_ARG;                \
+        }                                     \
+    } while (0)
+
+template <class T>
+static int dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        dh::set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        return DH_ERR_NOMEM;
+    }
+    return DH_OK;
+}
+#define DALLOC(p, n)                    \
+    do {                                \
+        int rc_ = dalloc(&(p), (n));    \
+        if (rc_) return rc_;            \
+    } while (0)
+
+static int round_ld(int N) { return ((N + TILE - 1) / TILE) * TILE; }
+
+extern "C" const char* dh_last_error(void) { return dh::last_error(); }
+extern "C" int dh_version(void) { return 100; }
+
+extern "C" int dh_init(int device, dh_context** out) {
+    CHECK_ARG(out != nullptr, "dh_init: null output");
+    int n = 0;
+    DH_HIP(hipGetDeviceCount(&n));
+    CHECK_ARG(device >= 0 && device < n, "dh_init: device index out of range");
+    DH_HIP(hipSetDevice(device));
+    dh_context* c = new dh_context();
+    c->device = device;
+    DH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (dalloc(&c->d_info, 1)) {
+        delete c;
+        return DH_ERR_NOMEM;
+    }
+    *out = c;
+    return DH_OK;
+}
+
+static void free_ws(dh_context* c) {
+    hipFree(c->S); hipFree(c->h); hipFree(c->W); hipFree(c->Hp); hipFree(c->Dinv); hipFree(c->trwork);
+    c->S = c->h = c->W = c->Hp = c->Dinv = c->trwork = nullptr;
+    c->ws_ld = 0;
+    c->ws_owner = nullptr;
+}
+
+static int ensure_ws(dh_context* c, int ld) {
+    if (c->ws_ld >= ld) return DH_OK;
+    DH_HIP(hipStreamSynchronize(c->stream));
+    free_ws(c);
+    const size_t n = (size_t)ld * ld;
+    DALLOC(c->S, n);
+    DALLOC(c->h, n);
+    DALLOC(c->W, n);
+    DALLOC(c->Hp, n);
+    DALLOC(c->Dinv, (size_t)ld * TILE);
+    DALLOC(c->trwork, (size_t)ld * TILE);
+    c->ws_ld = ld;
+    return DH_OK;
+}
+
+extern "C" int dh_system_destroy(dh_system* s) {
+    if (!s) return DH_OK;
+    if (s->ctx->ws_owner == s) s->ctx->ws_owner = nullptr;
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->ctx->stream);
+    void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out,
+                    s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->bounds, s->J, s->coef,
+                    s->kout, s->pops, s->erg, s->norms, s->norms0};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete s;
+    return DH_OK;
+}
+
+extern "C" int dh_finalize(dh_context* c) {
+    if (!c) return DH_OK;
+    hipSetDevice(c->device);
+    if (c->dropin) dh_system_destroy(c->dropin);
+    hipStreamSynchronize(c->stream);
+    free_ws(c);
+    hipFree(c->d_info);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return DH_OK;
+}
+
+extern "C" long long dh_system_bytes(int n_orb, int n_wp) {
+    const long long ld = round_ld(n_orb);
+    const long long R = 2LL * n_wp;
+    const int split = matvec_split_for(n_orb, 1);
+    long long b = 3LL * ld * ld * 8;                       // Q1, Q2, H_orth
+    b += (4 * R + 3LL * n_wp + n_wp) * ld * 16;             // phi, x0, x1, acc, out, hk
+    b += (long long)split * ld * 16 * 16;                  // partials
+    b += 3LL * ld * 16 + (long long)KMAX_ALLOC * 24;      // Lanczos, Bessel
+    return b;
+}
+
+extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_orb, const int* atom_lmax, int n_orb,
+                                const int* orb_n, const int* orb_l, const int* orb_m, const int* orb_nzeta,
+                                const double* orb_zeta, const double* orb_coef, const double* orb_ip,
+                                const double* orb_kwh, const double* orb_vshift, const int* orb_frag, int n_frag,
+                                int n_wp, dh_system** out) {
+    CHECK_ARG(ctx && out, "dh_system_create: null context or output");
+    CHECK_ARG(n_orb > 0 && n_wp > 0 && n_wp <= 64, "dh_system_create: need n_orb > 0 and 1 <= n_wp <= 64");
+    CHECK_ARG(n_frag >= 1 && n_frag <= 16, "dh_system_create: 1 <= n_frag <= 16");
+    CHECK_ARG(n_atoms >= 0, "dh_system_create: n_atoms < 0");
+    if (orb_frag)
+        for (int i = 0; i < n_orb; ++i)
+            CHECK_ARG(orb_frag[i] >= 0 && orb_frag[i] < n_frag, "dh_system_create: fragment index out of range");
+    DH_HIP(hipSetDevice(ctx->device));
+    std::vector<int> sh_n, sh_l, sh_nz, sh_orb0, at_ns, at_s0;
+    std::vector<double> sh_z, sh_c;
+    if (n_atoms > 0) {
+        CHECK_ARG(first_orb && atom_lmax && orb_n && orb_l && orb_m && orb_nzeta && orb_zeta && orb_coef && orb_ip &&
+                      orb_kwh && orb_vshift,
+                  "dh_system_create: basis arrays missing");
+        CHECK_ARG(first_orb[0] == 0 && first_orb[n_atoms] == n_orb, "dh_system_create: first_orb must span 0..n_orb");
+        for (int a = 0; a < n_atoms; ++a) {
+            at_s0.push_back((int)sh_n.size());
+            int k = first_orb[a], ns = 0;
+            CHECK_ARG(first_orb[a + 1] >= k, "dh_system_create: first_orb not monotone");
+            while (k < first_orb[a + 1]) {
+                const int l = orb_l[k];
+                CHECK_ARG(l >= 0 && l <= 2, "dh_system_create: only s, p, d shells (l <= 2) are supported");
+                CHECK_ARG(orb_n[k] >= 1 && orb_n[k] <= 5 && orb_n[k] > l, "dh_system_create: need l < n <= 5");
+                CHECK_ARG(orb_nzeta[k] >= 1 && orb_nzeta[k] <= 2, "dh_system_create: Nzeta must be 1 or 2");
+                CHECK_ARG(k + 2 * l + 1 <= first_orb[a + 1], "dh_system_create: incomplete shell");
+                for (int m = -l; m <= l; ++m)
+                    CHECK_ARG(orb_l[k + m + l] == l && orb_m[k + m + l] == m && orb_n[k + m + l] == orb_n[k],
+                              "dh_system_create: shell orbitals must be ordered m = -l..l (structure.f:211-223)");
+                sh_n.push_back(orb_n[k]);
+                sh_l.push_back(l);
+                sh_nz.push_back(orb_nzeta[k]);
+                sh_orb0.push_back(k);
+                sh_z.push_back(orb_zeta[2 * k]);
+                sh_z.push_back(orb_zeta[2 * k + 1]);
+                sh_c.push_back(orb_coef[2 * k]);
+                sh_c.push_back(orb_coef[2 * k + 1]);
+                k += 2 * l + 1;
+                ++ns;
+            }
+            CHECK_ARG(ns <= MAX_SHELLS, "dh_system_create: at most 3 shells per atom");
+            at_ns.push_back(ns);
+        }
+    }
+    dh_system* s = new dh_system();
+    s->ctx = ctx;
+    s->n_atoms = n_atoms;
+    s->N = n_orb;
+    s->ld = round_ld(n_orb);
+    s->W = n_wp;
+    s->n_frag = n_frag;
+    const int N = n_orb, ld = s->ld, R = 2 * n_wp;
+    int rc = DH_OK;
+#define SALLOC(p, n)                                   \
+    do {                                               \
+        if ((rc = dalloc(&(p), (n))) != DH_OK) {       \
+            dh_system_destroy(s);                      \
+            return rc;                                 \
+        }                                              \
+    } while (0)
+    // basis: one allocation, carved
+    const int ns = (int)sh_n.size();
+    const size_t bytes = (size_t)n_atoms * 2 * sizeof(int) + (size_t)ns * 4 * sizeof(int) +
+                         (size_t)ns * 4 * sizeof(double) + (size_t)N * 3 * sizeof(double) + 64 * 8;
+    char* bm = nullptr;
+    SALLOC(bm, bytes);
+    s->basis_mem = bm;
+    size_t off = 0;
+    auto carve = [&](size_t nbytes) {
+        char* p = bm + off;
+        off += (nbytes + 15) & ~size_t(15);
+        return p;
+    };
+    int* d_at_ns = (int*)carve(sizeof(int) * std::max(n_atoms, 1));
+    int* d_at_s0 = (int*)carve(sizeof(int) * std::max(n_atoms, 1));
+    int* d_sh_n = (int*)carve(sizeof(int) * std::max(ns, 1));
+    int* d_sh_l = (int*)carve(sizeof(int) * std::max(ns, 1));
+    int* d_sh_nz = (int*)carve(sizeof(int) * std::max(ns, 1));
+    int* d_sh_o = (int*)carve(sizeof(int) * std::max(ns, 1));
+    double* d_sh_z = (double*)carve(sizeof(double) * 2 * std::max(ns, 1));
+    double* d_sh_c = (double*)carve(sizeof(double) * 2 * std::max(ns, 1));
+    double* d_ip = (double*)carve(sizeof(double) * N);
+    double* d_kwh = (double*)carve(sizeof(double) * N);
+    double* d_vs = (double*)carve(sizeof(double) * N);
+    if (off > bytes) {
+        dh_system_destroy(s);
+        dh::set_error("dh_system_create: internal basis layout overflow");
+        return DH_ERR_ARG;
+    }
+    hipStream_t st = ctx->stream;
+    if (n_atoms > 0) {
+        DH_HIP(hipMemcpy(d_at_ns, at_ns.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_at_s0, at_s0.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_n, sh_n.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_l, sh_l.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_nz, sh_nz.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_o, sh_orb0.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_z, sh_z.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_sh_c, sh_c.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_ip, orb_ip, sizeof(double) * N, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_kwh, orb_kwh, sizeof(double) * N, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpy(d_vs, orb_vshift, sizeof(double) * N, hipMemcpyHostToDevice));
+        SALLOC(s->coords, (size_t)n_atoms * 3);
+    }
+    s->bd = BasisDev{n_atoms, s->coords, d_at_ns, d_at_s0, d_sh_n, d_sh_l, d_sh_nz, d_sh_o, d_sh_z, d_sh_c,
+                     d_ip, d_kwh, d_vs};
+    SALLOC(s->frag, N);
+    {
+        std::vector<int> fr(N, 0);
+        if (orb_frag)
+            for (int i = 0; i < N; ++i) fr[i] = orb_frag[i];
+        DH_HIP(hipMemcpy(s->frag, fr.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    }
+    const size_t mat = (size_t)ld * ld;
+    SALLOC(s->Q1, mat);
+    SALLOC(s->Q2, mat);
+    SALLOC(s->Hth, mat);
+    SALLOC(s->phi, (size_t)ld * R);
+    SALLOC(s->x0, (size_t)ld * R);
+    SALLOC(s->x1, (size_t)ld * R);
+    SALLOC(s->acc, (size_t)ld * R);
+    SALLOC(s->out, (size_t)ld * 3 * n_wp);
+    SALLOC(s->hk, (size_t)ld * n_wp);
+    s->split_max = matvec_split_for(N, 1);
+    SALLOC(s->partial, (size_t)s->split_max * ld * 16);
+    SALLOC(s->counters, (size_t)ld);
+    SALLOC(s->lv, (size_t)ld);
+    SALLOC(s->lvp, (size_t)ld);
+    SALLOC(s->lw, (size_t)ld);
+    SALLOC(s->lal, LANCZOS_MAX + 2);
+    SALLOC(s->lbe, LANCZOS_MAX + 2);
+    SALLOC(s->bounds, 8);
+    SALLOC(s->J, KMAX_ALLOC + 2);
+    SALLOC(s->coef, KMAX_ALLOC + 2);
+    SALLOC(s->kout, 4);
+    SALLOC(s->pops, (size_t)n_wp * (n_frag + 1));
+    SALLOC(s->erg, (size_t)n_wp * 2);
+    SALLOC(s->norms, (size_t)n_wp * 2);
+    SALLOC(s->norms0, (size_t)R);
+    launch_zero_u32(s->counters, ld, st);
+    launch_zero_cplx(s->out, (long)ld * 3 * n_wp, st);
+    launch_zero_cplx(s->phi, (long)ld * R, st);
+    launch_zero_cplx(s->hk, (long)ld * n_wp, st);
+    launch_zero_cplx(s->lw, (long)ld, st);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipStreamSynchronize(st));
+    *out = s;
+    return DH_OK;
+}
+
+extern "C" int dh_set_coords(dh_system* s, const double* coords) {
+    CHECK_ARG(s && coords, "dh_set_coords: null argument");
+    CHECK_ARG(s->n_atoms > 0, "dh_set_coords: operator-only system has no atoms");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    DH_HIP(hipMemcpyAsync(s->coords, coords, sizeof(double) * 3 * s->n_atoms, hipMemcpyHostToDevice, s->ctx->stream));
+    DH_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->have_coords = true;
+    return DH_OK;
+}
+
+// phi <- (L^-1 bra, L^T ket) from the AO state held in out[:, 0:W] and out[:, 2W:3W]
+static void convert_ao_to_phi(dh_system* s) {
+    hipStream_t st = s->ctx->stream;
+    const int W = s->W;
+    MatvecArgs a{};
+    a.ld = s->ld;
+    a.N = s->N;
+    a.partial = s->partial;
+    a.counters = s->counters;
+    a.epi = EPI_PLAIN;
+    a.R = W;
+    a.split = matvec_split_for(s->N, W);
+    // phi_bra = L^-1 bra : Q2 = sym(L^-1), j <= i
+    a.M = s->Q2;
+    a.tri = TRI_UPPER;
+    a.x = s->out; a.x_stride = 3 * W; a.x_off = 0;
+    a.out = s->phi; a.out_stride = 2 * W; a.out_off = 0;
+    launch_matvec(a, st);
+    // phi_ket = L^T ket : Q1 = sym(L), j >= i
+    a.M = s->Q1;
+    a.tri = TRI_LOWER;
+    a.x = s->out; a.x_stride = 3 * W; a.x_off = 2 * W;
+    a.out = s->phi; a.out_stride = 2 * W; a.out_off = W;
+    launch_matvec(a, st);
+}
+
+static int compute_bounds(dh_system* s) {
+    hipStream_t st = s->ctx->stream;
+    const int m = std::min(s->lanczos_steps, std::min(LANCZOS_MAX, s->N));
+    launch_lanczos_start(s->N, s->lv, s->lvp, 0x9e3779b9u, st);
+    MatvecArgs a{};
+    a.M = s->Hth;
+    a.ld = s->ld;
+    a.N = s->N;
+    a.R = 1;
+    a.partial = s->partial;
+    a.counters = s->counters;
+    a.split = matvec_split_for(s->N, 1);
+    a.tri = TRI_FULL;
+    a.epi = EPI_PLAIN;
+    a.x = s->lv; a.x_stride = 1; a.x_off = 0;
+    a.out = s->lw; a.out_stride = 1; a.out_off = 0;
+    for (int j = 0; j < m; ++j) {
+        launch_matvec(a, st);
+        launch_lanczos_step(s->N, s->lv, s->lvp, s->lw, s->lal, s->lbe, j, st);
+    }
+    launch_lanczos_bounds(s->lal, s->lbe, m, s->rel_margin, s->bounds, st);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipMemcpyAsync(s->bounds_h, s->bounds, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    s->emin = s->bounds_h[0];
+    s->emax = s->bounds_h[1];
+    s->widen = 1.0;
+    s->coef_dt = -1.0;  // force new coefficients
+    return DH_OK;
+}
+
+extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
+    CHECK_ARG(s, "dh_orthogonalize: null system");
+    dh_context* c = s->ctx;
+    if (c->ws_owner != s || c->ws_ld < s->ld) {
+        dh::set_error("dh_orthogonalize: S and h of this system are not loaded (call dh_build_operators or dh_load_operators)");
+        return DH_ERR_STATE;
+    }
+    DH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int ld = s->ld;
+    launch_copy_matrix(c->S, s->Q1, (long)ld * ld, st);
+    DH_HIP(hipMemsetAsync(c->d_info, 0, sizeof(int), st));
+    launch_cholesky(s->Q1, ld, c->Dinv, c->d_info, st);
+    int info = 0;
+    DH_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    if (info != 0) {
+        s->ops_ready = false;
+        dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
+        return DH_ERR_NOT_SPD;
+    }
+    launch_trtri_lower(s->Q1, s->Q2, c->Dinv, ld, c->trwork, st);
+    {   // W = L^-1 h
+        Gemm g{};
+        g.M = ld; g.N = ld; g.K = ld;
+        g.A = s->Q2; g.lda = ld;
+        g.B = c->h; g.ldb = ld;
+        g.C = c->W; g.ldc = ld;
+        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_ROWTILE; g.batch = 1;
+        launch_gemm(g, st);
+    }
+    {   // H_orth = W L^-T (lower tiles), then mirror
+        Gemm g{};
+        g.M = ld; g.N = ld; g.K = ld;
+        g.A = c->W; g.lda = ld;
+        g.B = s->Q2; g.ldb = ld; g.transB = 1;
+        g.C = s->Hth; g.ldc = ld;
+        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_COLTILE; g.lower_only = 1; g.batch = 1;
+        launch_gemm(g, st);
+        launch_symmetrize_from_lower(s->Hth, ld, st);
+    }
+    s->hprime_ready = false;
+    if (want_hprime) {  // H' = S^-1 h = L^-T W
+        Gemm g{};
+        g.M = ld; g.N = ld; g.K = ld;
+        g.A = s->Q2; g.lda = ld; g.transA = 1;
+        g.B = c->W; g.ldb = ld;
+        g.C = c->Hp; g.ldc = ld;
+        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_FROM_ROWTILE; g.batch = 1;
+        launch_gemm(g, st);
+        s->hprime_ready = true;
+    }
+    launch_symmetrize_from_lower(s->Q1, ld, st);
+    launch_symmetrize_from_lower(s->Q2, ld, st);
+    DH_HIP(hipGetLastError());
+    int rc = compute_bounds(s);
+    if (rc) return rc;
+    s->ops_ready = true;
+    s->phi_dirty = true;
+    return DH_OK;
+}
+
+extern "C" int dh_build_operators(dh_system* s, double cutoff, double s_drop) {
+    CHECK_ARG(s, "dh_build_operators: null system");
+    CHECK_ARG(s->n_atoms > 0 && s->have_coords, "dh_build_operators: coordinates not set");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    int rc = ensure_ws(s->ctx, s->ld);
+    if (rc) return rc;
+    s->ctx->ws_owner = s;
+    if (launch_build_SH(s->bd, s->N, s->ld, s->ctx->S, s->ctx->h, cutoff, s_drop, s->ctx->stream)) {
+        dh::set_error("dh_build_operators: kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        return DH_ERR_HIP;
+    }
+    return dh_orthogonalize(s, 0);
+}
+
+extern "C" int dh_load_operators(dh_system* s, const double* S, const double* h, int ldh) {
+    CHECK_ARG(s && S && h, "dh_load_operators: null argument");
+    CHECK_ARG(ldh >= s->N, "dh_load_operators: ld < N");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    int rc = ensure_ws(s->ctx, s->ld);
+    if (rc) return rc;
+    hipStream_t st = s->ctx->stream;
+    s->ctx->ws_owner = s;
+    launch_identity_pad(s->ctx->S, s->ctx->h, s->N, s->ld, st);
+    DH_HIP(hipMemcpy2DAsync(s->ctx->S, sizeof(double) * s->ld, S, sizeof(double) * ldh, sizeof(double) * s->N, s->N,
+                            hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpy2DAsync(s->ctx->h, sizeof(double) * s->ld, h, sizeof(double) * ldh, sizeof(double) * s->N, s->N,
+                            hipMemcpyHostToDevice, st));
+    return DH_OK;
+}
+
+extern "C" int dh_get_matrix(dh_system* s, int which, double* out, int ldo) {
+    CHECK_ARG(s && out && ldo >= s->N, "dh_get_matrix: bad argument");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    const double* src = nullptr;
+    const bool ws_ok = s->ctx->ws_owner == s && s->ctx->ws_ld >= s->ld;
+    switch (which) {
+        case DH_MAT_S: src = ws_ok ? s->ctx->S : nullptr; break;
+        case DH_MAT_H: src = ws_ok ? s->ctx->h : nullptr; break;
+        case DH_MAT_L: src = s->Q1; break;
+        case DH_MAT_LINV: src = s->Q2; break;
+        case DH_MAT_HORTH: src = s->Hth; break;
+        case DH_MAT_HPRIME: src = (s->hprime_ready && ws_ok) ? s->ctx->Hp : nullptr; break;
+        default: break;
+    }
+    if (!src) {
+        dh::set_error("dh_get_matrix: matrix %d not available", which);
+        return DH_ERR_STATE;
+    }
+    DH_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, src, sizeof(double) * s->ld, sizeof(double) * s->N, s->N,
+                            hipMemcpyDeviceToHost, s->ctx->stream));
+    DH_HIP(hipStreamSynchronize(s->ctx->stream));
+    return DH_OK;
+}
+
+extern "C" int dh_get_bounds(dh_system* s, double* out6) {
+    CHECK_ARG(s && out6, "dh_get_bounds: null argument");
+    CHECK_ARG(s->ops_ready, "dh_get_bounds: operators not built");
+    for (int i = 0; i < 6; ++i) out6[i] = s->bounds_h[i];
+    out6[0] = s->emin;
+    out6[1] = s->emax;
+    return DH_OK;
+}
+
+extern "C" int dh_set_lanczos(dh_system* s, int steps, double rel_margin) {
+    CHECK_ARG(s && steps >= 4 && steps <= LANCZOS_MAX && rel_margin >= 0.0, "dh_set_lanczos: bad argument");
+    s->lanczos_steps = steps;
+    s->rel_margin = rel_margin;
+    return DH_OK;
+}
+
+extern "C" int dh_set_wavepacket(dh_system* s, int wp, const double* bra, const double* ket) {
+    CHECK_ARG(s && bra && ket && wp >= 0 && wp < s->W, "dh_set_wavepacket: bad argument");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    const int W = s->W;
+    hipStream_t st = s->ctx->stream;
+    DH_HIP(hipMemcpy2DAsync(s->out + wp, sizeof(cplx) * 3 * W, bra, sizeof(cplx), sizeof(cplx), s->N,
+                            hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpy2DAsync(s->out + 2 * W + wp, sizeof(cplx) * 3 * W, ket, sizeof(cplx), sizeof(cplx), s->N,
+                            hipMemcpyHostToDevice, st));
+    DH_HIP(hipStreamSynchronize(st));
+    s->phi_dirty = true;
+    s->state_set = true;
+    return DH_OK;
+}
+
+extern "C" int dh_get_wavepacket(dh_system* s, int wp, double* bra, double* ket) {
+    CHECK_ARG(s && wp >= 0 && wp < s->W, "dh_get_wavepacket: bad argument");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    const int W = s->W;
+    hipStream_t st = s->ctx->stream;
+    if (bra)
+        DH_HIP(hipMemcpy2DAsync(bra, sizeof(cplx), s->out + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
+                                hipMemcpyDeviceToHost, st));
+    if (ket)
+        DH_HIP(hipMemcpy2DAsync(ket, sizeof(cplx), s->out + 2 * W + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
+                                hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    return DH_OK;
+}
+
+static int ensure_coefficients(dh_system* s, double dt) {
+    const double c = 0.5 * (s->emax + s->emin);
+    const double r = 0.5 * (s->emax - s->emin) * s->widen;
+    if (dt == s->coef_dt && c == s->coef_c && r == s->coef_r) return DH_OK;
+    hipStream_t st = s->ctx->stream;
+    launch_bessel(r * dt, c * dt, KMAX_ALLOC, 1e-18, s->J, s->coef, s->kout, st);
+    int k = 0;
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipMemcpyAsync(&k, s->kout, sizeof(int), hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    if (k <= 0 && r * dt > 1e-12) {
+        dh::set_error("Chebyshev coefficients: no terms for r*dt = %g", r * dt);
+        return DH_ERR_STATE;
+    }
+    s->K = k;
+    s->coef_dt = dt;
+    s->coef_c = c;
+    s->coef_r = r;
+    return DH_OK;
+}
+
+// One Chebyshev evaluation phi <- sum_k a_k T_k(Hh) phi, result in s->acc.
+static int chebyshev_series(dh_system* s) {
+    hipStream_t st = s->ctx->stream;
+    const int R = 2 * s->W;
+    const double c = s->coef_c, r = s->coef_r;
+    MatvecArgs a{};
+    a.M = s->Hth;
+    a.ld = s->ld;
+    a.N = s->N;
+    a.R = R;
+    a.partial = s->partial;
+    a.counters = s->counters;
+    a.split = matvec_split_for(s->N, R);
+    a.tri = TRI_FULL;
+    a.x_stride = R; a.x_off = 0;
+    a.out_stride = R; a.out_off = 0;
+    a.acc = s->acc;
+    a.coef = s->coef;
+    // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
+    a.epi = EPI_CHEB_FIRST;
+    a.x = s->phi;
+    a.out = s->x1;
+    a.alpha = 1.0 / r;
+    a.beta = -c / r;
+    a.k = 1;
+    launch_matvec(a, st);
+    cplx* prev2 = s->phi;
+    cplx* prev = s->x1;
+    for (int k = 2; k <= s->K; ++k) {
+        cplx* outb = (k == 2) ? s->x0 : prev2;
+        a.epi = EPI_CHEB;
+        a.x = prev;
+        a.x2 = prev2;
+        a.out = outb;
+        a.alpha = 2.0 / r;
+        a.beta = -2.0 * c / r;
+        a.k = k;
+        launch_matvec(a, st);
+        prev2 = prev;
+        prev = outb;
+    }
+    DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+static void launch_outputs(dh_system* s) {
+    hipStream_t st = s->ctx->stream;
+    const int W = s->W;
+    MatvecArgs a{};
+    a.ld = s->ld;
+    a.N = s->N;
+    a.partial = s->partial;
+    a.counters = s->counters;
+    a.epi = EPI_PLAIN;
+    // hk = H_orth phi_ket
+    a.M = s->Hth; a.tri = TRI_FULL; a.R = W; a.split = matvec_split_for(s->N, W);
+    a.x = s->phi; a.x_stride = 2 * W; a.x_off = W;
+    a.out = s->hk; a.out_stride = W; a.out_off = 0;
+    launch_matvec(a, st);
+    // bra = L phi_bra : Q1 = sym(L), j <= i
+    a.M = s->Q1; a.tri = TRI_UPPER; a.R = W;
+    a.x = s->phi; a.x_stride = 2 * W; a.x_off = 0;
+    a.out = s->out; a.out_stride = 3 * W; a.out_off = 0;
+    launch_matvec(a, st);
+    // [AO_bra | ket] = L^-T [phi_bra | phi_ket] : Q2 = sym(L^-1), j >= i
+    a.M = s->Q2; a.tri = TRI_LOWER; a.R = 2 * W; a.split = matvec_split_for(s->N, 2 * W);
+    a.x = s->phi; a.x_stride = 2 * W; a.x_off = 0;
+    a.out = s->out; a.out_stride = 3 * W; a.out_off = W;
+    launch_matvec(a, st);
+    launch_observables(s->N, W, s->n_frag, s->frag, s->out, s->phi, s->hk, s->pops, s->erg, s->norms, st);
+}
+
+extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_terms) {
+    CHECK_ARG(s, "dh_propagate: null system");
+    if (!s->ops_ready) {
+        dh::set_error("dh_propagate: operators not built");
+        return DH_ERR_STATE;
+    }
+    if (!s->state_set) {
+        dh::set_error("dh_propagate: wavepackets not set");
+        return DH_ERR_STATE;
+    }
+    DH_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const double dt = (t_max - t_init) / H_BAR;
+    const int R = 2 * s->W;
+    if (s->phi_dirty) {
+        convert_ao_to_phi(s);
+        s->phi_dirty = false;
+    }
+    if (!(dt > 0.0)) {
+        launch_outputs(s);
+        DH_HIP(hipStreamSynchronize(st));
+        if (n_terms) *n_terms = 0;
+        return DH_OK;
+    }
+    launch_column_norms(s->N, R, s->phi, s->norms0, st);
+    std::vector<double> n0(R), n1(2 * s->W);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        int rc = ensure_coefficients(s, dt);
+        if (rc) return rc;
+        rc = chebyshev_series(s);
+        if (rc) return rc;
+        std::swap(s->phi, s->acc);  // phi <- new state; acc keeps the old one
+        launch_outputs(s);
+        DH_HIP(hipMemcpyAsync(n0.data(), s->norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(n1.data(), s->norms, sizeof(double) * 2 * s->W, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        // n0 holds |phi_c|^2 per column (bra columns 0..W-1, ket columns W..2W-1)
+        double drift = 0.0;
+        for (int w = 0; w < s->W; ++w) {
+            const double b0 = n0[w], k0 = n0[s->W + w];
+            const double b1 = n1[2 * w], k1 = n1[2 * w + 1];
+            if (b0 > 0) drift = std::max(drift, std::fabs(b1 - b0) / b0);
+            if (k0 > 0) drift = std::max(drift, std::fabs(k1 - k0) / k0);
+            if (!std::isfinite(b1) || !std::isfinite(k1)) drift = INFINITY;
+        }
+        s->stats[4] = drift;
+        if (drift < 1e-9) {
+            s->stats[0] = s->K + 1;
+            s->stats[1] += 1;
+            s->stats[2] += s->K + 3;
+            if (n_terms) *n_terms = s->K + 1;
+            return DH_OK;
+        }
+        // the bounds did not enclose the spectrum: restore and widen
+        std::swap(s->phi, s->acc);
+        s->widen *= 1.25;
+        s->stats[3] += 1;
+    }
+    dh::set_error("dh_propagate: Chebyshev series lost the norm (drift %g) after widening the spectral bounds",
+                  s->stats[4]);
+    return DH_ERR_DIVERGED;
+}
+
+extern "C" int dh_get_observables(dh_system* s, int wp, double* ao_bra, double* pops, double* erg) {
+    CHECK_ARG(s && wp >= 0 && wp < s->W, "dh_get_observables: bad argument");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const int W = s->W;
+    if (ao_bra)
+        DH_HIP(hipMemcpy2DAsync(ao_bra, sizeof(cplx), s->out + W + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
+                                hipMemcpyDeviceToHost, st));
+    if (pops)
+        DH_HIP(hipMemcpyAsync(pops, s->pops + (size_t)wp * (s->n_frag + 1), sizeof(double) * (s->n_frag + 1),
+                              hipMemcpyDeviceToHost, st));
+    if (erg) DH_HIP(hipMemcpyAsync(erg, s->erg + 2 * wp, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    return DH_OK;
+}
+
+extern "C" int dh_get_stats(dh_system* s, double* out5) {
+    CHECK_ARG(s && out5, "dh_get_stats: null argument");
+    for (int i = 0; i < 5; ++i) out5[i] = s->stats[i];
+    return DH_OK;
+}
+
+extern "C" int dh_propagation_elhl(dh_context* ctx, const int* Np, const double* S, const double* h, double* H_out,
+                                   double* AO_bra, double* AO_ket, double* PSI_bra, double* PSI_ket,
+                                   const double* t_init, const double* t_max, double* tau, double* save_tau) {
+    (void)AO_ket;
+    CHECK_ARG(ctx && Np && S && h && PSI_bra && PSI_ket && t_init && t_max, "dh_propagation_elhl: null argument");
+    const int N = *Np;
+    CHECK_ARG(N > 0, "dh_propagation_elhl: N <= 0");
+    if (ctx->dropin && ctx->dropin_N != N) {
+        dh_system_destroy(ctx->dropin);
+        ctx->dropin = nullptr;
+    }
+    int rc;
+    if (!ctx->dropin) {
+        rc = dh_system_create(ctx, 0, nullptr, nullptr, N, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr, 1, 1, &ctx->dropin);
+        if (rc) return rc;
+        ctx->dropin_N = N;
+    }
+    dh_system* s = ctx->dropin;
+    if ((rc = dh_load_operators(s, S, h, N))) return rc;
+    if ((rc = dh_orthogonalize(s, H_out != nullptr))) return rc;
+    if (H_out && (rc = dh_get_matrix(s, DH_MAT_HPRIME, H_out, N))) return rc;
+    if ((rc = dh_set_wavepacket(s, 0, PSI_bra, PSI_ket))) return rc;
+    if ((rc = dh_propagate(s, *t_init, *t_max, nullptr))) return rc;
+    if ((rc = dh_get_wavepacket(s, 0, PSI_bra, PSI_ket))) return rc;
+    if (AO_bra && (rc = dh_get_observables(s, 0, AO_bra, nullptr, nullptr))) return rc;
+    const double step = (*t_max - *t_init) / H_BAR;
+    if (save_tau) *save_tau = step;
+    if (tau) *tau = step;
+    return DH_OK;
+}
+
+// ------------------------------------------------------------------ test hooks
+#include "../../include/dynemol_hip_testing.h"
+
+extern "C" int dh_test_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+                            int ldc, double alpha, double beta, int transA, int transB, int kmode, int lower_only) {
+    CHECK_ARG(M > 0 && N > 0 && K > 0 && A && B && C, "dh_test_gemm: bad argument");
+    const long na = (long)lda * (transA ? M : K), nb = (long)ldb * (transB ? K : N), nc = (long)ldc * N;
+    double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    DALLOC(dA, na);
+    DALLOC(dB, nb);
+    DALLOC(dC, nc);
+    DH_HIP(hipMemcpy(dA, A, sizeof(double) * na, hipMemcpyHostToDevice));
+    DH_HIP(hipMemcpy(dB, B, sizeof(double) * nb, hipMemcpyHostToDevice));
+    DH_HIP(hipMemcpy(dC, C, sizeof(double) * nc, hipMemcpyHostToDevice));
+    Gemm g{};
+    g.M = M; g.N = N; g.K = K;
+    g.A = dA; g.lda = lda; g.B = dB; g.ldb = ldb; g.C = dC; g.ldc = ldc;
+    g.alpha = alpha; g.beta = beta; g.transA = transA; g.transB = transB; g.kmode = kmode; g.lower_only = lower_only;
+    g.batch = 1;
+    launch_gemm(g, nullptr);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipDeviceSynchronize());
+    DH_HIP(hipMemcpy(C, dC, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    hipFree(dA); hipFree(dB); hipFree(dC);
+    return DH_OK;
+}
+
+extern "C" int dh_test_cholesky(int n, double* A, double* Linv, int* info) {
+    CHECK_ARG(n > 0 && n % TILE == 0 && A && Linv && info, "dh_test_cholesky: bad argument");
+    const long nn = (long)n * n;
+    double *dA = nullptr, *dX = nullptr, *dD = nullptr, *dw = nullptr;
+    int* di = nullptr;
+    DALLOC(dA, nn);
+    DALLOC(dX, nn);
+    DALLOC(dD, (long)n * TILE);
+    DALLOC(dw, (long)n * TILE);
+    DALLOC(di, 1);
+    DH_HIP(hipMemcpy(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice));
+    DH_HIP(hipMemset(di, 0, sizeof(int)));
+    launch_cholesky(dA, n, dD, di, nullptr);
+    launch_trtri_lower(dA, dX, dD, n, dw, nullptr);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipDeviceSynchronize());
+    DH_HIP(hipMemcpy(A, dA, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    DH_HIP(hipMemcpy(Linv, dX, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    DH_HIP(hipMemcpy(info, di, sizeof(int), hipMemcpyDeviceToHost));
+    hipFree(dA); hipFree(dX); hipFree(dD); hipFree(dw); hipFree(di);
+    return DH_OK;
+}

@@ -1,0 +1,503 @@
+// Chebyshev exp(-i H dt) propagation kernels (FP64), spectral bounds and
+// observables.
+//
+// The reference propagates with an adaptive-tau Taylor series, streaming H'
+// twice per term (bra and ket GEMVs: Taylor.f:90-100, Taylor_gpu.cpp:411-416)
+// with a host synchronisation per term.  Here every wavepacket's bra and ket
+// are right-hand-side columns of one real-symmetric H_orth, and one kernel
+// launch per Chebyshev term streams H_orth once for all of them:
+//
+//   T_0 = phi, T_1 = Hh phi, T_k = 2 Hh T_{k-1} - T_{k-2},  Hh = (H - c)/r
+//   phi(t + dt) = sum_k a_k T_k,  a_k = (2 - delta_k0) (-i)^k J_k(r dt) e^{-i c dt}
+//
+// matvec_kernel: y_i = sum_j M[j + ld*i] x_j (column i of the column-major
+// matrix is contiguous, so lanes stream it coalesced).  Grid = column blocks x
+// row splits; each workgroup leaves a partial sum per column, and the last
+// workgroup of a column block (ticket counter) adds the partials in split
+// order and applies the fused epilogue.  The summation order is fixed, so
+// results are bitwise reproducible run to run.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace dh {
+
+constexpr int MV_WAVES = 4;
+
+template <int R, int CPW>
+__global__ __launch_bounds__(256) void matvec_kernel(MatvecArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ncol = MV_WAVES * CPW;
+    const int i_base = blockIdx.x * ncol + wave * CPW;
+    const int s = blockIdx.y;
+    const int rows_per = (((a.N + a.split - 1) / a.split) + 63) & ~63;
+    int j_begin = s * rows_per;
+    int j_end = min(a.N, j_begin + rows_per);
+    if (a.tri == TRI_LOWER) j_begin = max(j_begin, (i_base / 64) * 64);  // j >= i (masked per element below)
+    if (a.tri == TRI_UPPER) j_end = min(j_end, i_base + CPW);             // j <= i
+
+    double yr[CPW][R], yi[CPW][R];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            yr[c][r] = 0.0;
+            yi[c][r] = 0.0;
+        }
+    const long ld = a.ld;
+    int ic[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) ic[c] = min(i_base + c, a.N - 1);  // clamped: loads stay in bounds
+
+    for (int j0 = j_begin; j0 < j_end; j0 += 128) {
+        double m[2][CPW];
+        double xr[2][R], xi[2][R];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int j = j0 + u * 64 + lane;
+            const bool ok = j < j_end;
+            const int jj = ok ? j : j_begin;
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                double v = a.M[jj + ld * ic[c]];
+                const int i = i_base + c;
+                bool keep = ok && i < a.N;
+                if (a.tri == TRI_LOWER) keep = keep && (jj >= i);
+                if (a.tri == TRI_UPPER) keep = keep && (jj <= i);
+                m[u][c] = keep ? v : 0.0;
+            }
+            const cplx* xp = a.x + (long)jj * a.x_stride + a.x_off;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const cplx xv = xp[r];
+                xr[u][r] = ok ? xv.re : 0.0;
+                xi[u][r] = ok ? xv.im : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int c = 0; c < CPW; ++c)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    yr[c][r] = fma(m[u][c], xr[u][r], yr[c][r]);
+                    yi[c][r] = fma(m[u][c], xi[u][r], yi[c][r]);
+                }
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            yr[c][r] = wave_sum(yr[c][r]);
+            yi[c][r] = wave_sum(yi[c][r]);
+        }
+
+    __shared__ int s_last;
+    __shared__ cplx s_y[MV_WAVES * CPW * R];
+    if (a.split > 1) {
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int i = i_base + c;
+                if (i < a.N)
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        a.partial[((long)s * a.N + i) * R + r] = cplx{yr[c][r], yi[c][r]};
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned ticket = atomicAdd(&a.counters[blockIdx.x], 1u);
+            s_last = (ticket == (unsigned)(a.split - 1));
+        }
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        // sum partials in split order
+        for (int e = threadIdx.x; e < ncol * R; e += blockDim.x) {
+            const int c = e / R, r = e % R;
+            const int i = blockIdx.x * ncol + c;
+            cplx y{0.0, 0.0};
+            if (i < a.N)
+                for (int ss = 0; ss < a.split; ++ss) {
+                    const cplx p = a.partial[((long)ss * a.N + i) * R + r];
+                    y.re += p.re;
+                    y.im += p.im;
+                }
+            s_y[e] = y;
+        }
+        if (threadIdx.x == 0) a.counters[blockIdx.x] = 0u;
+    } else {
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < CPW; ++c)
+#pragma unroll
+                for (int r = 0; r < R; ++r) s_y[(wave * CPW + c) * R + r] = cplx{yr[c][r], yi[c][r]};
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ncol * R; e += blockDim.x) {
+        const int c = e / R, r = e % R;
+        const int i = blockIdx.x * ncol + c;
+        if (i >= a.N) continue;
+        const cplx y = s_y[e];
+        const long oi = (long)i * a.out_stride + a.out_off + r;
+        if (a.epi == EPI_PLAIN) {
+            a.out[oi] = y;
+        } else {
+            const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+            cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+            if (a.epi == EPI_CHEB) {
+                const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+                T.re -= x2.re;
+                T.im -= x2.im;
+                const cplx add = cmul(a.coef[a.k], T);
+                cplx acc = a.acc[oi];
+                acc.re += add.re;
+                acc.im += add.im;
+                a.acc[oi] = acc;
+            } else {
+                const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+                a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+            }
+            a.out[oi] = T;
+        }
+    }
+}
+
+int matvec_split_for(int N, int R) {
+    const int cpw = R <= 4 ? 4 : (R <= 8 ? 2 : 1);
+    const int colblocks = (N + MV_WAVES * cpw - 1) / (MV_WAVES * cpw);
+    int split = (1024 + colblocks - 1) / colblocks;
+    const int maxsplit = (N + 255) / 256;
+    if (split > maxsplit) split = maxsplit;
+    if (split < 1) split = 1;
+    const int rows_per = (((N + split - 1) / split) + 63) & ~63;
+    split = (N + rows_per - 1) / rows_per;
+    return split;
+}
+
+template <int R, int CPW>
+static void launch_mv(const MatvecArgs& a, hipStream_t st) {
+    const int colblocks = (a.N + MV_WAVES * CPW - 1) / (MV_WAVES * CPW);
+    hipLaunchKernelGGL((matvec_kernel<R, CPW>), dim3(colblocks, a.split), dim3(256), 0, st, a);
+}
+
+void launch_matvec(const MatvecArgs& a, hipStream_t st) {
+    switch (a.R) {
+        case 1: launch_mv<1, 4>(a, st); break;
+        case 2: launch_mv<2, 4>(a, st); break;
+        case 3: launch_mv<3, 4>(a, st); break;
+        case 4: launch_mv<4, 4>(a, st); break;
+        case 6: launch_mv<6, 2>(a, st); break;
+        case 8: launch_mv<8, 2>(a, st); break;
+        case 12: launch_mv<12, 1>(a, st); break;
+        case 16: launch_mv<16, 1>(a, st); break;
+        default: {
+            // generic: process in chunks of up to 16 columns
+            int done = 0;
+            while (done < a.R) {
+                int take = a.R - done;
+                int t = take >= 16 ? 16 : take >= 12 ? 12 : take >= 8 ? 8 : take >= 6 ? 6 : take >= 4 ? 4 : take >= 3 ? 3 : take >= 2 ? 2 : 1;
+                MatvecArgs b = a;
+                b.R = t;
+                b.x_off = a.x_off + done;
+                b.out_off = a.out_off + done;
+                launch_matvec(b, st);
+                done += t;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Lanczos
+// One Lanczos step on a single workgroup (1024 threads):
+//   alpha_j = v.w ; w -= alpha_j v + beta_j v_prev ; beta_{j+1} = |w| ;
+//   v_prev = v ; v = w / beta_{j+1}.
+// w is the real part of a complex buffer (stride 1 complex).
+__device__ double block_sum_1024(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < 64) {
+        t = threadIdx.x < (int)(blockDim.x >> 6) ? sh[threadIdx.x] : 0.0;
+        t = wave_sum(t);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh[0] = t;
+    __syncthreads();
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void lanczos_step_kernel(int N, cplx* v, cplx* vprev, const cplx* w, double* alpha,
+                                                            double* beta, int j) {
+    __shared__ double sh[64];
+    double p = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) p += v[i].re * w[i].re;
+    const double al = block_sum_1024(p, sh);
+    const double be = j > 0 ? beta[j] : 0.0;
+    double q = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double wi = w[i].re - al * v[i].re - be * vprev[i].re;
+        q += wi * wi;
+    }
+    const double nb = sqrt(block_sum_1024(q, sh));
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double wi = w[i].re - al * v[i].re - be * vprev[i].re;
+        const double vi = v[i].re;
+        vprev[i] = cplx{vi, 0.0};
+        v[i] = cplx{nb > 0.0 ? wi / nb : 0.0, 0.0};
+    }
+    if (threadIdx.x == 0) {
+        alpha[j] = al;
+        beta[j + 1] = nb;
+    }
+}
+
+__global__ void lanczos_start_kernel(int N, cplx* v, cplx* vprev, unsigned seed) {
+    // deterministic pseudo-random start vector, normalised below by the first step's norm
+    __shared__ double sh[64];
+    double q = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        unsigned x = (unsigned)i * 2654435761u ^ seed;
+        x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+        const double r = ((double)x / 4294967296.0) - 0.5;
+        v[i] = cplx{r, 0.0};
+        vprev[i] = cplx{0.0, 0.0};
+        q += r * r;
+    }
+    const double nrm = sqrt(block_sum_1024(q, sh));
+    for (int i = threadIdx.x; i < N; i += blockDim.x) v[i].re /= nrm;
+}
+
+// Sturm count: number of eigenvalues of the m x m tridiagonal below x.
+__device__ int sturm_count(const double* al, const double* be, int m, double x) {
+    int cnt = 0;
+    double q = 1.0;
+    for (int k = 0; k < m; ++k) {
+        const double b2 = k > 0 ? be[k] * be[k] : 0.0;
+        q = (al[k] - x) - (k > 0 ? b2 / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0.0) ++cnt;
+    }
+    return cnt;
+}
+
+// Extreme Ritz values by bisection, residual of each extreme Ritz pair
+// (|beta_m s_m|, s from inverse iteration on the tridiagonal), and the
+// enclosing interval [emin, emax] = [theta_min - margin_lo, theta_max + margin_hi].
+__global__ void lanczos_bounds_kernel(const double* al, const double* be, int m, double rel_margin, double* bounds) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo = 1e300, hi = -1e300;
+    for (int k = 0; k < m; ++k) {
+        const double r = (k > 0 ? fabs(be[k]) : 0.0) + (k + 1 < m ? fabs(be[k + 1]) : 0.0);
+        lo = fmin(lo, al[k] - r);
+        hi = fmax(hi, al[k] + r);
+    }
+    double th[2];
+    for (int e = 0; e < 2; ++e) {
+        const int target = e == 0 ? 1 : m;  // smallest: count(x) >= 1 ; largest: count(x) >= m
+        double a = lo, b = hi;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (a + b);
+            if (sturm_count(al, be, m, mid) >= target) b = mid; else a = mid;
+            if (b - a <= 1e-13 * fmax(1.0, fabs(mid))) break;
+        }
+        th[e] = 0.5 * (a + b);
+    }
+    double res[2];
+    for (int e = 0; e < 2; ++e) {
+        // inverse iteration with a slightly shifted theta on the tridiagonal
+        const double shift = th[e] + (e == 0 ? -1e-10 : 1e-10) * fmax(1.0, fabs(th[e]));
+        double s[128], dd[128], uu[128];
+        for (int k = 0; k < m; ++k) s[k] = 1.0;
+        for (int it = 0; it < 3; ++it) {
+            // LU of (T - shift I) without pivoting (Thomas algorithm)
+            dd[0] = al[0] - shift;
+            for (int k = 1; k < m; ++k) {
+                const double l = be[k] / (fabs(dd[k - 1]) > 1e-300 ? dd[k - 1] : 1e-300);
+                dd[k] = al[k] - shift - l * be[k];
+                uu[k] = l;
+            }
+            for (int k = 1; k < m; ++k) s[k] -= uu[k] * s[k - 1];
+            s[m - 1] /= (fabs(dd[m - 1]) > 1e-300 ? dd[m - 1] : 1e-300);
+            for (int k = m - 2; k >= 0; --k) s[k] = (s[k] - be[k + 1] * s[k + 1]) / (fabs(dd[k]) > 1e-300 ? dd[k] : 1e-300);
+            double nrm = 0.0;
+            for (int k = 0; k < m; ++k) nrm += s[k] * s[k];
+            nrm = sqrt(nrm);
+            for (int k = 0; k < m; ++k) s[k] /= nrm;
+        }
+        res[e] = fabs(be[m] * s[m - 1]);
+    }
+    const double width = th[1] - th[0];
+    bounds[0] = th[0] - res[0] - rel_margin * width;
+    bounds[1] = th[1] + res[1] + rel_margin * width;
+    bounds[2] = th[0];
+    bounds[3] = th[1];
+    bounds[4] = res[0];
+    bounds[5] = res[1];
+}
+
+// ------------------------------------------------------------------ Bessel coefficients
+// J_k(x), k = 0..K, by Miller's backward recurrence J_{k-1} = (2k/x) J_k - J_{k+1}
+// normalised with J_0 + 2 sum J_{2k} = 1; K = last k with |J_k| > tol.
+// coef[k] = (2 - delta_k0) (-i)^k J_k(x) e^{-i c dt};  out_k[0] = K (number of terms - 1).
+__global__ void bessel_coef_kernel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int K = 0;
+    if (x < 1e-12) {
+        J[0] = 1.0;
+        K = 0;
+    } else {
+        int M = (int)(x + 15.0 * cbrt(x) + 40.0);
+        M += M & 1;
+        if (M > kmax_alloc - 2) M = kmax_alloc - 2;
+        double jp1 = 0.0, jk = 1e-300;
+        for (int k = 0; k <= M + 1; ++k) J[k] = 0.0;
+        J[M] = jk;
+        for (int k = M; k >= 1; --k) {
+            const double jm1 = (2.0 * k / x) * jk - jp1;
+            J[k - 1] = jm1;
+            jp1 = jk;
+            jk = jm1;
+            if (fabs(jk) > 1e250) {
+                for (int q = k - 1; q <= M; ++q) J[q] *= 1e-250;
+                jk *= 1e-250;
+                jp1 *= 1e-250;
+            }
+        }
+        double norm = J[0];
+        for (int k = 2; k <= M; k += 2) norm += 2.0 * J[k];
+        for (int k = 0; k <= M; ++k) J[k] /= norm;
+        K = M;
+        while (K > 0 && fabs(J[K]) < tol) --K;
+        K += 1;
+        if (K > M) K = M;
+    }
+    const double cp = cos(phase), sp = -sin(phase);  // e^{-i phase}
+    for (int k = 0; k <= K; ++k) {
+        const double f = (k == 0 ? 1.0 : 2.0) * J[k];
+        // (-i)^k
+        double re, im;
+        switch (k & 3) {
+            case 0: re = f; im = 0.0; break;
+            case 1: re = 0.0; im = -f; break;
+            case 2: re = -f; im = 0.0; break;
+            default: re = 0.0; im = f; break;
+        }
+        coef[k] = cplx{re * cp - im * sp, re * sp + im * cp};
+    }
+    coef[K + 1] = cplx{0.0, 0.0};
+    out_k[0] = K;
+}
+
+// ------------------------------------------------------------------ observables
+// One workgroup per wavepacket w: fragment populations
+//   pop_f = Re sum_{i in f} conj(ket_i) bra_i  (data_output.f:242-263, DUAL_bra = conj(ket), DUAL_ket = bra)
+// and <H> = sum_i conj(phi_bra_i) (H_orth phi_ket)_i (ElHl_Chebyshev.f:329-371).
+// out: [bra (W) | AO_bra (W) | ket (W)] per row; phi: [bra (W) | ket (W)]; hk: N x W.
+__global__ __launch_bounds__(256) void observables_kernel(int N, int W, int n_frag, const int* frag, const cplx* out,
+                                                          const cplx* phi, const cplx* hk, double* pops, double* erg,
+                                                          double* norms) {
+    constexpr int MAXF = 16;
+    __shared__ double sh[(MAXF + 5) * 4];
+    const int w = blockIdx.x;
+    double pf[MAXF + 1];
+    for (int f = 0; f <= MAXF; ++f) pf[f] = 0.0;
+    double er = 0.0, ei = 0.0, nk = 0.0, nb = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const cplx b = out[(long)i * 3 * W + w];
+        const cplx k = out[(long)i * 3 * W + 2 * W + w];
+        const double p = k.re * b.re + k.im * b.im;  // Re(conj(k) b)
+        const int f = frag[i];
+        if (f >= 0 && f < MAXF) pf[f] += p;
+        pf[MAXF] += p;
+        const cplx pb = phi[(long)i * 2 * W + w];
+        const cplx pk = phi[(long)i * 2 * W + W + w];
+        const cplx hv = hk[(long)i * W + w];
+        er += pb.re * hv.re + pb.im * hv.im;
+        ei += pb.re * hv.im - pb.im * hv.re;
+        nb += pb.re * pb.re + pb.im * pb.im;
+        nk += pk.re * pk.re + pk.im * pk.im;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    auto red = [&](double v, int slot) {
+        v = wave_sum(v);
+        if (lane == 0) sh[slot * 4 + wv] = v;
+    };
+    for (int f = 0; f < n_frag && f < MAXF; ++f) red(pf[f], f);
+    red(pf[MAXF], MAXF);
+    red(er, MAXF + 1);
+    red(ei, MAXF + 2);
+    red(nb, MAXF + 3);
+    red(nk, MAXF + 4);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        auto tot = [&](int slot) { return ((sh[slot * 4 + 0] + sh[slot * 4 + 1]) + sh[slot * 4 + 2]) + sh[slot * 4 + 3]; };
+        for (int f = 0; f < n_frag && f < MAXF; ++f) pops[w * (n_frag + 1) + f] = tot(f);
+        pops[w * (n_frag + 1) + n_frag] = tot(MAXF);
+        erg[2 * w] = tot(MAXF + 1);
+        erg[2 * w + 1] = tot(MAXF + 2);
+        norms[2 * w] = tot(MAXF + 3);
+        norms[2 * w + 1] = tot(MAXF + 4);
+    }
+}
+
+// Column norms |phi_c|^2 of an N x R complex block (one workgroup per column).
+__global__ __launch_bounds__(256) void column_norms_kernel(int N, int R, const cplx* x, double* out) {
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const cplx v = x[(long)i * R + c];
+        s += v.re * v.re + v.im * v.im;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+__global__ void zero_cplx_kernel(cplx* p, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = cplx{0.0, 0.0};
+}
+
+
+// ------------------------------------------------------------------ launchers
+void launch_zero_u32(unsigned* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(64), dim3(256), 0, st, p, n);
+}
+void launch_zero_cplx(cplx* p, long n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_cplx_kernel, dim3(256), dim3(256), 0, st, p, n);
+}
+void launch_lanczos_start(int N, cplx* v, cplx* vprev, unsigned seed, hipStream_t st) {
+    hipLaunchKernelGGL(lanczos_start_kernel, dim3(1), dim3(1024), 0, st, N, v, vprev, seed);
+}
+void launch_lanczos_step(int N, cplx* v, cplx* vprev, const cplx* w, double* alpha, double* beta, int j,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, st, N, v, vprev, w, alpha, beta, j);
+}
+void launch_lanczos_bounds(const double* al, const double* be, int m, double rel_margin, double* bounds,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(lanczos_bounds_kernel, dim3(1), dim3(64), 0, st, al, be, m, rel_margin, bounds);
+}
+void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(bessel_coef_kernel, dim3(1), dim3(64), 0, st, x, phase, kmax_alloc, tol, J, coef, out_k);
+}
+void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi, const cplx* hk,
+                        double* pops, double* erg, double* norms, hipStream_t st) {
+    hipLaunchKernelGGL(observables_kernel, dim3(W), dim3(256), 0, st, N, W, n_frag, frag, out, phi, hk, pops, erg,
+                       norms);
+}
+void launch_column_norms(int N, int R, const cplx* x, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(column_norms_kernel, dim3(R), dim3(256), 0, st, N, R, x, out);
+}
+
+}  // namespace dh

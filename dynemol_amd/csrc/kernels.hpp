@@ -1,0 +1,97 @@
+// Kernel launchers shared between the translation units of the library.
+#pragma once
+#include "common.hpp"
+
+namespace dh {
+
+constexpr int MAX_SHELLS = 3;  // s, p, d
+constexpr int TILE = 64;       // dense-algebra block size; ld is padded to a multiple of it
+
+// Device view of the basis (struct-of-arrays; all pointers are device memory).
+struct BasisDev {
+    int n_atoms;
+    const double* coords;     // [n_atoms*3], x y z per atom, Angstrom
+    const int* atom_nshell;   // [n_atoms]
+    const int* atom_shell0;   // [n_atoms] first shell index
+    const int* sh_n;          // [n_shells]
+    const int* sh_l;
+    const int* sh_nzeta;
+    const int* sh_orb0;       // first orbital of the shell (m = -l)
+    const double* sh_zeta;    // [2*n_shells], 1/Angstrom
+    const double* sh_coef;    // [2*n_shells]
+    const double* ip;         // [N] per orbital
+    const double* kwh;        // [N]
+    const double* vshift;     // [N]
+};
+
+int launch_build_SH(const BasisDev& b, int N, int ld, double* S, double* h, double cutoff, double s_drop,
+                    hipStream_t st);
+
+// ---------------------------------------------------------------- dense FP64
+enum KMode { K_FULL = 0, K_BELOW_ROWTILE = 1, K_BELOW_COLTILE = 2, K_FROM_ROWTILE = 3 };
+
+struct Gemm {
+    int M, N, K;
+    const double* A;
+    long lda, strideA;
+    const double* B;
+    long ldb, strideB;
+    double* C;
+    long ldc, strideC;
+    double alpha, beta;
+    int transA, transB;
+    int kmode;       // KMode: restrict k to where a triangular operand is nonzero
+    int lower_only;  // compute only tiles with row-tile >= col-tile
+    int batch;
+};
+void launch_gemm(const Gemm& g, hipStream_t st);
+
+// Cholesky S = L L^T in place (lower; upper zeroed) on an ld x ld padded
+// matrix, plus L^-1 in Linv.  info (device int) gets the first failing
+// column + 1, or stays 0.  Dinv: (ld/64) blocks of 64x64 workspace.
+void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st);
+void launch_trtri_lower(const double* L, double* Linv, const double* Dinv, int ld, double* work, hipStream_t st);
+void launch_symmetrize_from_lower(double* A, int ld, hipStream_t st);
+void launch_copy_matrix(const double* src, double* dst, long n, hipStream_t st);
+
+// ---------------------------------------------------------------- propagation
+enum Tri { TRI_FULL = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
+enum Epi { EPI_PLAIN = 0, EPI_CHEB_FIRST = 1, EPI_CHEB = 2 };
+
+struct MatvecArgs {
+    const double* M;  // column-major ld x ld; y_i = sum_j M[j + ld*i] x_j (= (M^T x)_i)
+    int ld, N;
+    int R;            // complex columns of x (row-major N x R)
+    const cplx* x;    // T_{k-1} (or the input)
+    const cplx* x2;   // T_{k-2}
+    cplx* out;        // T_k or y
+    cplx* acc;        // accumulator
+    cplx* partial;    // [split][N][R]
+    unsigned* counters;
+    int split;
+    int tri;
+    int epi;
+    double alpha, beta;     // T = alpha*y + beta*x - x2 (EPI_CHEB), alpha*y + beta*x (FIRST)
+    const cplx* coef;       // device Chebyshev coefficients a_0..a_K (and a_{K+1} = 0)
+    int k;                  // term index for EPI_CHEB
+    int x_stride, x_off;    // column selection: uses x[i*x_stride + x_off + r], r < R
+    int out_stride, out_off;
+};
+int matvec_split_for(int N, int R);
+void launch_matvec(const MatvecArgs& a, hipStream_t st);
+
+void launch_zero_u32(unsigned* p, int n, hipStream_t st);
+void launch_zero_cplx(cplx* p, long n, hipStream_t st);
+void launch_lanczos_start(int N, cplx* v, cplx* vprev, unsigned seed, hipStream_t st);
+void launch_lanczos_step(int N, cplx* v, cplx* vprev, const cplx* w, double* alpha, double* beta, int j,
+                         hipStream_t st);
+void launch_lanczos_bounds(const double* al, const double* be, int m, double rel_margin, double* bounds,
+                           hipStream_t st);
+void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k,
+                   hipStream_t st);
+void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi, const cplx* hk,
+                        double* pops, double* erg, double* norms, hipStream_t st);
+void launch_column_norms(int N, int R, const cplx* x, double* out, hipStream_t st);
+void launch_identity_pad(double* S, double* h, int N, int ld, hipStream_t st);
+
+}  // namespace dh

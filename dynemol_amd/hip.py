@@ -1,0 +1,271 @@
+"""ctypes binding of libdynemol_hip.so (the C ABI declared in include/dynemol_hip.h).
+
+There is no CPU fallback: if the library is missing or no GPU is visible, the
+calls raise.  The classes mirror the reference's per-step sequence:
+`System.build` ~ Overlap_Matrix + Build_Huckel (+ orthogonalisation),
+`System.propagate` ~ Propagation for every wavepacket (Taylor.f:35-127),
+`System.observables` ~ QuasiParticleEnergies + Populations.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes
+import os
+import sys
+import weakref
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdynemol_hip.so")
+
+DH_MAT_S, DH_MAT_H, DH_MAT_L, DH_MAT_LINV, DH_MAT_HORTH, DH_MAT_HPRIME = range(6)
+H_BAR = 6.58264e-4
+
+_lib = None
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+SYMBOLS = [
+    "dh_last_error", "dh_version", "dh_init", "dh_finalize", "dh_system_create", "dh_system_destroy",
+    "dh_system_bytes", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
+    "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
+    "dh_get_observables", "dh_get_stats", "dh_propagation_elhl",
+]
+TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky"]
+
+# Every live context and system is released before interpreter shutdown, while
+# the HIP runtime is still up (a handle freed during finalisation aborts).
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for obj in sorted(list(_LIVE), key=lambda o: 0 if isinstance(o, System) else 1):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+class DHError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DHError(f"{LIB_PATH} is not built; run `python -m dynemol_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.dh_last_error.restype = ctypes.c_char_p
+    L.dh_init.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.dh_finalize.argtypes = [_vp]
+    L.dh_system_create.argtypes = [_vp, ctypes.c_int, _i, _i, ctypes.c_int, _i, _i, _i, _i, _d, _d, _d, _d, _d, _i,
+                                   ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.dh_system_destroy.argtypes = [_vp]
+    L.dh_system_bytes.restype = ctypes.c_longlong
+    L.dh_system_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.dh_set_coords.argtypes = [_vp, _d]
+    L.dh_build_operators.argtypes = [_vp, ctypes.c_double, ctypes.c_double]
+    L.dh_load_operators.argtypes = [_vp, _d, _d, ctypes.c_int]
+    L.dh_orthogonalize.argtypes = [_vp, ctypes.c_int]
+    L.dh_get_matrix.argtypes = [_vp, ctypes.c_int, _d, ctypes.c_int]
+    L.dh_get_bounds.argtypes = [_vp, _d]
+    L.dh_set_lanczos.argtypes = [_vp, ctypes.c_int, ctypes.c_double]
+    L.dh_set_wavepacket.argtypes = [_vp, ctypes.c_int, _d, _d]
+    L.dh_get_wavepacket.argtypes = [_vp, ctypes.c_int, _d, _d]
+    L.dh_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
+    L.dh_get_observables.argtypes = [_vp, ctypes.c_int, _d, _d, _d]
+    L.dh_get_stats.argtypes = [_vp, _d]
+    L.dh_propagation_elhl.argtypes = [_vp, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]
+    L.dh_test_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int, _d,
+                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int]
+    L.dh_test_cholesky.argtypes = [ctypes.c_int, _d, _d, _i]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().dh_last_error().decode(errors="replace")
+        raise DHError(f"{what} failed ({rc}): {msg}")
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = _vp()
+        _check(lib().dh_init(device, ctypes.byref(self.h)), "dh_init")
+        self.systems = weakref.WeakSet()
+        _LIVE.add(self)
+
+    def close(self):
+        if self.h:
+            for s in list(self.systems):
+                s.close()
+            lib().dh_finalize(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def propagation_elhl(self, S, h, psi_bra, psi_ket, t_init, t_max, want_hprime=True):
+        """propagationelhl_gpucaller_ semantics; returns (psi_bra, psi_ket, ao_bra, H_prime, tau, save_tau)."""
+        N = S.shape[0]
+        S = np.asfortranarray(S, dtype=np.float64)
+        h = np.asfortranarray(h, dtype=np.float64)
+        bra = np.array(psi_bra, dtype=np.complex128, copy=True)
+        ket = np.array(psi_ket, dtype=np.complex128, copy=True)
+        ao = np.zeros(N, dtype=np.complex128)
+        Hp = np.zeros((N, N), order="F") if want_hprime else None
+        n = ctypes.c_int(N)
+        ti, tm = ctypes.c_double(t_init), ctypes.c_double(t_max)
+        tau, st = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(lib().dh_propagation_elhl(self.h, ctypes.byref(n), S.ctypes.data_as(_d), h.ctypes.data_as(_d),
+                                         Hp.ctypes.data_as(_d) if Hp is not None else None,
+                                         ao.ctypes.data_as(_d), None, bra.ctypes.data_as(_d), ket.ctypes.data_as(_d),
+                                         ctypes.byref(ti), ctypes.byref(tm), ctypes.byref(tau), ctypes.byref(st)),
+               "dh_propagation_elhl")
+        return bra, ket, ao, Hp, tau.value, st.value
+
+
+class System:
+    """Device-resident operators and `n_wp` wavepackets of one molecular system."""
+
+    def __init__(self, ctx: Context, basis=None, n_wp: int = 2, n_frag: int = 1, n_orb: int | None = None):
+        self.ctx = ctx
+        self.h = _vp()
+        _LIVE.add(self)
+        ctx.systems.add(self)
+        self.n_wp = n_wp
+        self.n_frag = n_frag
+        if basis is None:
+            assert n_orb is not None
+            self.N = int(n_orb)
+            self.n_atoms = 0
+            _check(lib().dh_system_create(ctx.h, 0, None, None, self.N, None, None, None, None, None, None, None,
+                                          None, None, None, n_frag, n_wp, ctypes.byref(self.h)), "dh_system_create")
+            return
+        b = basis
+        self.basis = b
+        self.N = b.N
+        self.n_atoms = b.n_atoms
+        self._keep = [np.ascontiguousarray(x) for x in (b.first_orb, b.atom_lmax, b.n, b.l, b.m, b.nzeta, b.zeta,
+                                                         b.coef, b.ip, b.k_wh, b.v_shift, b.fragment)]
+        fo, lm, n, l, m, nz, z, c, ip, kw, vs, fr = self._keep
+        fr = fr.astype(np.int32)
+        self._keep[-1] = fr
+        _check(lib().dh_system_create(ctx.h, b.n_atoms, _p(fo, _i), _p(lm, _i), b.N, _p(n, _i), _p(l, _i), _p(m, _i),
+                                      _p(nz, _i), _p(z), _p(c), _p(ip), _p(kw), _p(vs), _p(fr, _i), n_frag, n_wp,
+                                      ctypes.byref(self.h)), "dh_system_create")
+
+    def close(self):
+        if self.h:
+            lib().dh_system_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- operators
+    def set_coords(self, coords):
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        assert x.shape == (self.n_atoms, 3)
+        _check(lib().dh_set_coords(self.h, _p(x)), "dh_set_coords")
+
+    def build(self, coords=None, cutoff=12.0, s_drop=0.0):
+        if coords is not None:
+            self.set_coords(coords)
+        _check(lib().dh_build_operators(self.h, cutoff, s_drop), "dh_build_operators")
+
+    def load_operators(self, S, h, want_hprime=False):
+        S = np.asfortranarray(S, dtype=np.float64)
+        h = np.asfortranarray(h, dtype=np.float64)
+        _check(lib().dh_load_operators(self.h, S.ctypes.data_as(_d), h.ctypes.data_as(_d), S.shape[0]),
+               "dh_load_operators")
+        _check(lib().dh_orthogonalize(self.h, 1 if want_hprime else 0), "dh_orthogonalize")
+
+    def matrix(self, which):
+        out = np.zeros((self.N, self.N), order="F")
+        _check(lib().dh_get_matrix(self.h, which, out.ctypes.data_as(_d), self.N), "dh_get_matrix")
+        return out
+
+    def bounds(self):
+        out = np.zeros(6)
+        _check(lib().dh_get_bounds(self.h, _p(out)), "dh_get_bounds")
+        return out
+
+    def set_lanczos(self, steps, rel_margin):
+        _check(lib().dh_set_lanczos(self.h, steps, rel_margin), "dh_set_lanczos")
+
+    # -- wavepackets
+    def set_wavepacket(self, wp, psi_bra, psi_ket):
+        b = np.ascontiguousarray(psi_bra, dtype=np.complex128)
+        k = np.ascontiguousarray(psi_ket, dtype=np.complex128)
+        assert b.shape == (self.N,) and k.shape == (self.N,)
+        _check(lib().dh_set_wavepacket(self.h, wp, b.ctypes.data_as(_d), k.ctypes.data_as(_d)), "dh_set_wavepacket")
+
+    def wavepacket(self, wp):
+        b = np.zeros(self.N, dtype=np.complex128)
+        k = np.zeros(self.N, dtype=np.complex128)
+        _check(lib().dh_get_wavepacket(self.h, wp, b.ctypes.data_as(_d), k.ctypes.data_as(_d)), "dh_get_wavepacket")
+        return b, k
+
+    def propagate(self, t_init, t_max):
+        n = ctypes.c_int(0)
+        _check(lib().dh_propagate(self.h, t_init, t_max, ctypes.byref(n)), "dh_propagate")
+        return n.value
+
+    def observables(self, wp):
+        ao = np.zeros(self.N, dtype=np.complex128)
+        pops = np.zeros(self.n_frag + 1)
+        erg = np.zeros(2)
+        _check(lib().dh_get_observables(self.h, wp, ao.ctypes.data_as(_d), _p(pops), _p(erg)), "dh_get_observables")
+        return ao, pops, complex(erg[0], erg[1])
+
+    def stats(self):
+        out = np.zeros(5)
+        _check(lib().dh_get_stats(self.h, _p(out)), "dh_get_stats")
+        return {"terms": int(out[0]), "steps": int(out[1]), "matvecs": int(out[2]), "retries": int(out[3]),
+                "norm_drift": float(out[4])}
+
+
+def run_gemm(A, B, C, alpha=1.0, beta=0.0, transA=0, transB=0, kmode=0, lower_only=0):
+    """Runs the library's FP64 MFMA GEMM kernel on host operands (column-major)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    M, N = C.shape
+    K = A.shape[0] if transA else A.shape[1]
+    _check(lib().dh_test_gemm(M, N, K, A.ctypes.data_as(_d), A.shape[0], B.ctypes.data_as(_d), B.shape[0],
+                              C.ctypes.data_as(_d), M, alpha, beta, transA, transB, kmode, lower_only), "dh_test_gemm")
+    return C
+
+
+def run_cholesky(S):
+    A = np.array(S, dtype=np.float64, order="F", copy=True)
+    n = A.shape[0]
+    X = np.zeros_like(A)
+    info = ctypes.c_int(0)
+    _check(lib().dh_test_cholesky(n, A.ctypes.data_as(_d), X.ctypes.data_as(_d), ctypes.byref(info)),
+           "dh_test_cholesky")
+    return A, X, info.value
+
+
+def system_bytes(n_orb, n_wp):
+    return int(lib().dh_system_bytes(n_orb, n_wp))

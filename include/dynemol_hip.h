@@ -1,0 +1,120 @@
+/*
+ * dynemol_hip: MI355X (gfx950) electron/hole wavepacket propagator.
+ *
+ * C ABI: plain pointers and sizes, int status returns (0 = success,
+ * negative = error; dh_last_error() describes the last failure of the
+ * calling thread's most recent call).  Matrices are column-major (Fortran
+ * order); complex vectors are interleaved (re, im) double pairs, i.e.
+ * Fortran complex*16.  Host pointers are plain pageable or pinned memory.
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not
+ * overlap.  Systems created on a context run on that context's stream.
+ */
+#ifndef DYNEMOL_HIP_H
+#define DYNEMOL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DH_OK = 0,
+    DH_ERR_ARG = -1,      /* invalid argument or unsupported basis */
+    DH_ERR_HIP = -2,      /* HIP runtime error */
+    DH_ERR_NOT_SPD = -3,  /* Cholesky of S failed: S not positive definite */
+    DH_ERR_DIVERGED = -4, /* Chebyshev series lost the norm after widening the bounds */
+    DH_ERR_NOMEM = -5,    /* device allocation failed */
+    DH_ERR_STATE = -6     /* call out of order (e.g. propagate before operators) */
+};
+
+enum { DH_MAT_S = 0, DH_MAT_H = 1, DH_MAT_L = 2, DH_MAT_LINV = 3, DH_MAT_HORTH = 4, DH_MAT_HPRIME = 5 };
+
+typedef struct dh_context dh_context;
+typedef struct dh_system dh_system;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dh_last_error(void);
+int dh_version(void);
+
+/* Replaces gpu_init_ (GPU_Interface.cpp:226-263): select the device, create
+ * the stream.  The reference picks the device round-robin by MPI rank; here
+ * the caller passes it. */
+int dh_init(int device, dh_context **ctx);
+/* Replaces gpu_finalize_ (GPU_Interface.cpp:265-285). */
+int dh_finalize(dh_context *ctx);
+
+/*
+ * A system: device-resident operators and wavepackets of one molecule.
+ * The basis arguments are the reference's STO_basis record in struct-of-
+ * arrays form (types_EHT.f:114-166; built by Basis_Builder, structure.f:
+ * 228-324): per orbital n, l, m, Nzeta, zeta(2) [1/Angstrom], coef(2), IP
+ * [eV], k_WH, V_shift, fragment index; per atom the first orbital
+ * (BasisPointer) and AngMax.  n_atoms = 0 creates an operator-only system
+ * (S and h loaded from the host).  n_wp wavepackets (electron, hole, ...)
+ * each carry a bra and a ket (Psi_t_bra / Psi_t_ket, ElHl_Chebyshev.f:40).
+ */
+int dh_system_create(dh_context *ctx, int n_atoms, const int *first_orb, const int *atom_lmax, int n_orb,
+                     const int *orb_n, const int *orb_l, const int *orb_m, const int *orb_nzeta,
+                     const double *orb_zeta, const double *orb_coef, const double *orb_ip, const double *orb_kwh,
+                     const double *orb_vshift, const int *orb_frag, int n_frag, int n_wp, dh_system **sys);
+int dh_system_destroy(dh_system *sys);
+/* Device bytes a system of this size holds (serving admission). */
+long long dh_system_bytes(int n_orb, int n_wp);
+
+/* Atomic coordinates, n_atoms x 3 row-major (x, y, z per atom), Angstrom
+ * (system%coord, types_EHT.f).  */
+int dh_set_coords(dh_system *sys, const double *coords);
+/* Overlap S and Hueckel h from the coordinates, on the device
+ * (Overlap_Matrix overlap_D.f:31-106 + Build_Huckel ElHl_Chebyshev.f:296-323),
+ * then dh_orthogonalize.  cutoff: QM_cutoff (12 A, constants_m.f:34).
+ * s_drop > 0 zeroes off-diagonal |S_ij| < s_drop as the reference's
+ * unmoved-atom recycle cache does (overlap_D.f:193-225, 586-655). */
+int dh_build_operators(dh_system *sys, double cutoff, double s_drop);
+/* Host S and h (N x N, leading dimension ld), then dh_orthogonalize: the
+ * operands propagationelhl_gpucaller_ receives (Taylor_gpu.cpp:634-682). */
+int dh_load_operators(dh_system *sys, const double *S, const double *h, int ld);
+/* S = L L^T, L^-1, H_orth = L^-1 h L^-T, Lanczos spectral bounds; with
+ * want_hprime also H' = S^-1 h (replaces ElHl_Chebyshev.f:205-210 and
+ * GPU_Interface.cpp:880-949 + Taylor_gpu.cpp:687-696). */
+int dh_orthogonalize(dh_system *sys, int want_hprime);
+/* Copy a device matrix (DH_MAT_*) to host, N x N with leading dimension ld. */
+int dh_get_matrix(dh_system *sys, int which, double *out, int ld);
+/* [E_min, E_max, theta_min, theta_max, residual_min, residual_max] (eV). */
+int dh_get_bounds(dh_system *sys, double *out6);
+/* Override the Lanczos step count and relative safety margin (defaults 48, 0.02). */
+int dh_set_lanczos(dh_system *sys, int steps, double rel_margin);
+
+/* Wavepacket wp's AO-basis Psi_t_bra and Psi_t_ket, N complex each. */
+int dh_set_wavepacket(dh_system *sys, int wp, const double *psi_bra, const double *psi_ket);
+int dh_get_wavepacket(dh_system *sys, int wp, double *psi_bra, double *psi_ket);
+/*
+ * Propagate every wavepacket from t_init to t_max (ps): Psi_ket <- exp(-i
+ * S^-1 h dt / hbar) Psi_ket and Psi_bra <- exp(-i h S^-1 dt / hbar) Psi_bra,
+ * the operators Propagation applies (Taylor.f:35-127).  n_terms (may be
+ * NULL) receives the Chebyshev term count used.
+ */
+int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
+/* Observables after dh_propagate for wavepacket wp:
+ * ao_bra (N complex) = S^-1 Psi_t_bra, not conjugated (Taylor_gpu.cpp:716-718);
+ * pops (n_frag + 1) = Populations: Re sum_frag conj(Psi_ket) Psi_bra, total
+ * last (data_output.f:87-147); erg (2) = QuasiParticleEnergies
+ * sum_ij conj(ao_bra)_i h_ij Psi_ket_j (ElHl_Chebyshev.f:329-371). */
+int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, double *erg);
+/* Counters: [terms of last step, steps, matvecs, divergence retries, last norm drift]. */
+int dh_get_stats(dh_system *sys, double *out5);
+
+/*
+ * Drop-in for propagationelhl_gpucaller_ (Taylor_gpu.cpp:219-232, 634-736):
+ * host S, h (N x N, lda N) -> H_out = S^-1 h (may be NULL), PSI_bra/PSI_ket
+ * propagated in place from *t_init to *t_max, AO_bra = S^-1 PSI_bra (not
+ * conjugated), AO_ket unused, *tau and *save_tau set to the step in eV^-1.
+ * All arguments by reference like the reference's implicit Fortran interface.
+ */
+int dh_propagation_elhl(dh_context *ctx, const int *N, const double *S, const double *h, double *H_out,
+                        double *AO_bra, double *AO_ket, double *PSI_bra, double *PSI_ket, const double *t_init,
+                        const double *t_max, double *tau, double *save_tau);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
