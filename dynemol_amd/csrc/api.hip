@@ -71,7 +71,13 @@ struct dh_system {
     double widen = 1.0;
     double coef_dt = -1.0, coef_c = 0.0, coef_r = 0.0;
     int K = 0;
-    double stats[5] = {0, 0, 0, 0, 0};
+    // [0] terms of last step, [1] steps, [2] matvec launches, [3] divergence retries,
+    // [4] last norm drift, [5] term-series GPU ms (events), [6] term launches timed,
+    // [7] operator rebuilds, [8] rebuild wall ms
+    double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool timing = false;
+    bool tuned = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 #define CHECK_ARG(cond, msg)                  \
@@ -148,6 +154,8 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->ctx->ws_owner == s) s->ctx->ws_owner = nullptr;
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->bounds, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0};
@@ -323,6 +331,8 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->erg, (size_t)n_wp * 2);
     SALLOC(s->norms, (size_t)n_wp * 2);
     SALLOC(s->norms0, (size_t)R);
+    DH_HIP(hipEventCreate(&s->ev0));
+    DH_HIP(hipEventCreate(&s->ev1));
     launch_zero_u32(s->counters, ld, st);
     launch_zero_cplx(s->out, (long)ld * 3 * n_wp, st);
     launch_zero_cplx(s->phi, (long)ld * R, st);
@@ -401,6 +411,72 @@ static int compute_bounds(dh_system* s) {
     return DH_OK;
 }
 
+// Project autotuner: time every matvec kernel variant on this system's real
+// operands for each shape the propagation uses, keep the fastest.  Variants
+// differ only in tile shape and reduction tree (all FP64, all deterministic).
+static int autotune_shape(dh_system* s, const MatvecArgs& base) {
+    hipStream_t st = s->ctx->stream;
+    int best = -1;
+    float best_ms = 1e30f;
+    const int nv = matvec_variant_count();
+    for (int v = 0; v < nv; ++v) {
+        for (int w = 0; w < 2; ++w) launch_matvec_variant(base, v, st);
+        DH_HIP(hipEventRecord(s->ev0, st));
+        const int reps = 6;
+        for (int r = 0; r < reps; ++r) launch_matvec_variant(base, v, st);
+        DH_HIP(hipEventRecord(s->ev1, st));
+        DH_HIP(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        DH_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    matvec_set_tuned(base.N, base.R, base.tri, best);
+    return DH_OK;
+}
+
+static int autotune_system(dh_system* s) {
+    const int W = s->W, R = 2 * W;
+    MatvecArgs a{};
+    a.ld = s->ld;
+    a.N = s->N;
+    a.partial = s->partial;
+    a.counters = s->counters;
+    a.epi = EPI_PLAIN;
+    a.x_off = 0;
+    a.out_off = 0;
+    struct Shape {
+        const double* M;
+        int tri, R;
+        cplx *x, *out;
+        int xs, os;
+    } shapes[] = {
+        {s->Hth, TRI_FULL, R, s->phi, s->x0, R, R},   // Chebyshev terms
+        {s->Hth, TRI_FULL, W, s->phi, s->x1, R, W},   // <H> matvec
+        {s->Q1, TRI_UPPER, W, s->phi, s->x1, R, W},   // L phi_bra
+        {s->Q2, TRI_LOWER, R, s->phi, s->x0, R, R},   // L^-T phi
+        {s->Q1, TRI_LOWER, W, s->phi, s->x1, R, W},   // L^T ket
+        {s->Q2, TRI_UPPER, W, s->phi, s->x1, R, W},   // L^-1 bra
+        {s->Hth, TRI_FULL, 1, s->lv, s->lw, 1, 1},    // Lanczos
+    };
+    for (auto& sh : shapes) {
+        a.M = sh.M;
+        a.tri = sh.tri;
+        a.R = sh.R;
+        a.x = sh.x;
+        a.out = sh.out;
+        a.x_stride = sh.xs;
+        a.out_stride = sh.os;
+        a.split = matvec_split_for(s->N, sh.R);
+        int rc = autotune_shape(s, a);
+        if (rc) return rc;
+    }
+    s->tuned = true;
+    return DH_OK;
+}
+
 extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
     CHECK_ARG(s, "dh_orthogonalize: null system");
     dh_context* c = s->ctx;
@@ -456,6 +532,10 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
     launch_symmetrize_from_lower(s->Q1, ld, st);
     launch_symmetrize_from_lower(s->Q2, ld, st);
     DH_HIP(hipGetLastError());
+    if (!s->tuned) {
+        int trc = autotune_system(s);
+        if (trc) return trc;
+    }
     int rc = compute_bounds(s);
     if (rc) return rc;
     s->ops_ready = true;
@@ -465,6 +545,7 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
 
 extern "C" int dh_build_operators(dh_system* s, double cutoff, double s_drop) {
     CHECK_ARG(s, "dh_build_operators: null system");
+    s->stats[7] += 1;
     CHECK_ARG(s->n_atoms > 0 && s->have_coords, "dh_build_operators: coordinates not set");
     DH_HIP(hipSetDevice(s->ctx->device));
     int rc = ensure_ws(s->ctx, s->ld);
@@ -685,8 +766,10 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
     for (int attempt = 0; attempt < 4; ++attempt) {
         int rc = ensure_coefficients(s, dt);
         if (rc) return rc;
+        if (s->timing) DH_HIP(hipEventRecord(s->ev0, st));
         rc = chebyshev_series(s);
         if (rc) return rc;
+        if (s->timing) DH_HIP(hipEventRecord(s->ev1, st));
         std::swap(s->phi, s->acc);  // phi <- new state; acc keeps the old one
         launch_outputs(s);
         DH_HIP(hipMemcpyAsync(n0.data(), s->norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
@@ -702,6 +785,12 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
             if (!std::isfinite(b1) || !std::isfinite(k1)) drift = INFINITY;
         }
         s->stats[4] = drift;
+        if (s->timing) {
+            float ms = 0.0f;
+            DH_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            s->stats[5] += ms;
+            s->stats[6] += s->K;
+        }
         if (drift < 1e-9) {
             s->stats[0] = s->K + 1;
             s->stats[1] += 1;
@@ -735,9 +824,15 @@ extern "C" int dh_get_observables(dh_system* s, int wp, double* ao_bra, double* 
     return DH_OK;
 }
 
-extern "C" int dh_get_stats(dh_system* s, double* out5) {
-    CHECK_ARG(s && out5, "dh_get_stats: null argument");
-    for (int i = 0; i < 5; ++i) out5[i] = s->stats[i];
+extern "C" int dh_get_stats(dh_system* s, double* out9) {
+    CHECK_ARG(s && out9, "dh_get_stats: null argument");
+    for (int i = 0; i < 9; ++i) out9[i] = s->stats[i];
+    return DH_OK;
+}
+
+extern "C" int dh_set_timing(dh_system* s, int on) {
+    CHECK_ARG(s, "dh_set_timing: null system");
+    s->timing = on != 0;
     return DH_OK;
 }
 
@@ -820,5 +915,90 @@ extern "C" int dh_test_cholesky(int n, double* A, double* Linv, int* info) {
     DH_HIP(hipMemcpy(Linv, dX, sizeof(double) * nn, hipMemcpyDeviceToHost));
     DH_HIP(hipMemcpy(info, di, sizeof(int), hipMemcpyDeviceToHost));
     hipFree(dA); hipFree(dX); hipFree(dD); hipFree(dw); hipFree(di);
+    return DH_OK;
+}
+
+__global__ void test_copy_kernel(const double* __restrict__ a, double* __restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+extern "C" int dh_test_copy(long long bytes, int repeats, double* ms_per_copy) {
+    CHECK_ARG(bytes >= 8 && repeats >= 1, "dh_test_copy: bad argument");
+    const long n = (long)(bytes / 8);
+    double *a = nullptr, *b = nullptr;
+    DALLOC(a, n);
+    DALLOC(b, n);
+    DH_HIP(hipMemset(a, 0, sizeof(double) * n));
+    hipEvent_t e0, e1;
+    DH_HIP(hipEventCreate(&e0));
+    DH_HIP(hipEventCreate(&e1));
+    DH_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(test_copy_kernel, dim3(4096), dim3(256), 0, nullptr, a, b, n);
+    DH_HIP(hipEventRecord(e1, nullptr));
+    DH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_copy) *ms_per_copy = ms / repeats;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return DH_OK;
+}
+
+extern "C" int dh_test_force_matvec_variant(int variant) {
+    CHECK_ARG(variant >= -1 && variant < matvec_variant_count(), "dh_test_force_matvec_variant: bad variant");
+    matvec_force_variant(variant);
+    return DH_OK;
+}
+
+extern "C" int dh_matvec_tuning(dh_system* s, int* out7) {
+    CHECK_ARG(s && out7, "dh_matvec_tuning: null argument");
+    const int W = s->W, R = 2 * W;
+    const int q[7][2] = {{R, TRI_FULL}, {W, TRI_FULL}, {W, TRI_UPPER}, {R, TRI_LOWER}, {W, TRI_LOWER}, {W, TRI_UPPER}, {1, TRI_FULL}};
+    for (int k = 0; k < 7; ++k) out7[k] = matvec_variant_for(s->N, q[k][0], q[k][1]);
+    return DH_OK;
+}
+
+extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double* us_per_launch) {
+    CHECK_ARG(N > 0 && R >= 1 && R <= 16 && variant >= 0 && variant < matvec_variant_count() && reps > 0,
+              "dh_test_matvec_timing: bad argument");
+    const int ld = round_ld(N);
+    double* M = nullptr;
+    cplx *x = nullptr, *x2 = nullptr, *out = nullptr, *acc = nullptr, *coef = nullptr, *partial = nullptr;
+    unsigned* counters = nullptr;
+    const int split = matvec_split_for(N, R);
+    DALLOC(M, (size_t)ld * ld);
+    DALLOC(x, (size_t)ld * R);
+    DALLOC(x2, (size_t)ld * R);
+    DALLOC(out, (size_t)ld * R);
+    DALLOC(acc, (size_t)ld * R);
+    DALLOC(coef, 8);
+    DALLOC(partial, (size_t)split * ld * 16);
+    DALLOC(counters, (size_t)ld);
+    DH_HIP(hipMemset(M, 0, sizeof(double) * (size_t)ld * ld));
+    DH_HIP(hipMemset(x, 0, sizeof(cplx) * (size_t)ld * R));
+    DH_HIP(hipMemset(x2, 0, sizeof(cplx) * (size_t)ld * R));
+    DH_HIP(hipMemset(acc, 0, sizeof(cplx) * (size_t)ld * R));
+    DH_HIP(hipMemset(coef, 0, sizeof(cplx) * 8));
+    DH_HIP(hipMemset(counters, 0, sizeof(unsigned) * ld));
+    MatvecArgs a{};
+    a.M = M; a.ld = ld; a.N = N; a.R = R; a.x = x; a.x2 = x2; a.out = out; a.acc = acc; a.partial = partial;
+    a.counters = counters; a.split = split; a.tri = TRI_FULL; a.epi = EPI_CHEB; a.alpha = 1.0; a.beta = 0.0;
+    a.coef = coef; a.k = 2; a.x_stride = R; a.x_off = 0; a.out_stride = R; a.out_off = 0;
+    hipEvent_t e0, e1;
+    DH_HIP(hipEventCreate(&e0));
+    DH_HIP(hipEventCreate(&e1));
+    for (int w = 0; w < 3; ++w) launch_matvec_variant(a, variant, nullptr);
+    DH_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) launch_matvec_variant(a, variant, nullptr);
+    DH_HIP(hipEventRecord(e1, nullptr));
+    DH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_launch = 1e3 * ms / reps;
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    hipFree(M); hipFree(x); hipFree(x2); hipFree(out); hipFree(acc); hipFree(coef); hipFree(partial); hipFree(counters);
     return DH_OK;
 }

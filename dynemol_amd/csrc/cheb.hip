@@ -19,6 +19,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace dh {
 
 constexpr int MV_WAVES = 4;
@@ -164,6 +167,283 @@ __global__ __launch_bounds__(256) void matvec_kernel(MatvecArgs a) {
     }
 }
 
+// Column-block matvec: a workgroup of NW waves owns NW*CPW consecutive output
+// columns over the whole j range, so no other workgroup contributes to them
+// (no partial sums, no fences).  x (T_{k-1}) is staged through a
+// double-buffered LDS tile of JC rows, stored as separate re/im planes per
+// column so lanes read consecutive doubles; the next tile is fetched into
+// registers while the current one is consumed.
+// PIPE = 0: let the compiler schedule the tile; 1: read x one row ahead and
+// fence the scheduler per row (bounds register use); 2: fence per row, no lookahead.
+template <int R, int CPW, int NW, int JC, int PIPE = 0>
+__global__ __launch_bounds__(NW * 64) void matvec_cols_kernel(MatvecArgs a) {
+    static_assert(JC % 64 == 0 && JC >= 64, "tile height must be a multiple of 64");
+    static_assert(CPW * R <= 64, "one lane per output in the epilogue");
+    constexpr int NT = NW * 64;
+    constexpr int TE = JC * R;                 // complex elements per tile
+    constexpr int PER = (TE + NT - 1) / NT;    // complex elements per thread per tile
+    constexpr int TU = JC / 64;                // rows per lane per tile
+    __shared__ double xs[2][2 * R][JC];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int NC = NW * CPW;
+    const int c0 = blockIdx.x * NC;
+    const int i_base = c0 + wave * CPW;
+    int jb = 0, je = a.N;
+    if (a.tri == TRI_LOWER) jb = (c0 / JC) * JC;
+    if (a.tri == TRI_UPPER) je = min(a.N, c0 + NC);
+    const long ld = a.ld;
+    int ic[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) ic[c] = min(i_base + c, a.N - 1);
+    double yr[CPW][R], yi[CPW][R];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            yr[c][r] = 0.0;
+            yi[c][r] = 0.0;
+        }
+    cplx pre[PER];
+    auto fetch = [&](int t0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + q * NT;
+            const int j = t0 + e / R, r = e % R;
+            cplx v{0.0, 0.0};
+            if (e < TE && j < je) v = a.x[(long)j * a.x_stride + a.x_off + r];
+            pre[q] = v;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + q * NT;
+            if (e < TE) {
+                const int jl = e / R, r = e % R;
+                xs[buf][2 * r][jl] = pre[q].re;
+                xs[buf][2 * r + 1][jl] = pre[q].im;
+            }
+        }
+    };
+    if (jb < je) {
+        fetch(jb);
+        stash(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int t0 = jb; t0 < je; t0 += JC) {
+        const bool more = t0 + JC < je;
+        if (more) fetch(t0 + JC);
+        double hv[TU][CPW];
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int j = t0 + u * 64 + lane;
+            const int jj = j < je ? j : jb;
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const double v = a.M[jj + ld * ic[c]];
+                const int i = i_base + c;
+                bool keep = j < je && i < a.N;
+                if (a.tri == TRI_LOWER) keep = keep && (j >= i);
+                if (a.tri == TRI_UPPER) keep = keep && (j <= i);
+                hv[u][c] = keep ? v : 0.0;
+            }
+        }
+if constexpr (PIPE == 0) {
+#pragma unroll
+            for (int u = 0; u < TU; ++u) {
+                const int jl = u * 64 + lane;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double xr = xs[buf][2 * r][jl], xi = xs[buf][2 * r + 1][jl];
+#pragma unroll
+                    for (int c = 0; c < CPW; ++c) {
+                        yr[c][r] = fma(hv[u][c], xr, yr[c][r]);
+                        yi[c][r] = fma(hv[u][c], xi, yi[c][r]);
+                    }
+                }
+            }
+        } else {
+            double xc[2 * R], xn[2 * R];
+#pragma unroll
+            for (int q = 0; q < 2 * R; ++q) xc[q] = xs[buf][q][lane];
+#pragma unroll
+            for (int u = 0; u < TU; ++u) {
+                if constexpr (PIPE == 1) {
+                    if (u + 1 < TU) {
+#pragma unroll
+                        for (int q = 0; q < 2 * R; ++q) xn[q] = xs[buf][q][(u + 1) * 64 + lane];
+                    }
+                } else if (u > 0) {
+#pragma unroll
+                    for (int q = 0; q < 2 * R; ++q) xc[q] = xs[buf][q][u * 64 + lane];
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int c = 0; c < CPW; ++c) {
+                        yr[c][r] = fma(hv[u][c], xc[2 * r], yr[c][r]);
+                        yi[c][r] = fma(hv[u][c], xc[2 * r + 1], yi[c][r]);
+                    }
+                __builtin_amdgcn_sched_barrier(0);
+                if constexpr (PIPE == 1) {
+#pragma unroll
+                    for (int q = 0; q < 2 * R; ++q) xc[q] = xn[q];
+                }
+            }
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // reduce over lanes; lane (c*R + r) then owns output (i_base + c, r)
+    double my_re = 0.0, my_im = 0.0;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double sr = wave_sum(yr[c][r]);
+            const double si = wave_sum(yi[c][r]);
+            if (lane == c * R + r) {
+                my_re = sr;
+                my_im = si;
+            }
+        }
+    if (lane >= CPW * R) return;
+    const int c = lane / R, r = lane % R;
+    const int i = i_base + c;
+    if (i >= a.N) return;
+    const cplx y{my_re, my_im};
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    if (a.epi == EPI_PLAIN) {
+        a.out[oi] = y;
+        return;
+    }
+    const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+    cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+    if (a.epi == EPI_CHEB) {
+        const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+        T.re -= x2.re;
+        T.im -= x2.im;
+        const cplx add = cmul(a.coef[a.k], T);
+        cplx acc = a.acc[oi];
+        acc.re += add.re;
+        acc.im += add.im;
+        a.acc[oi] = acc;
+    } else {
+        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+    a.out[oi] = T;
+}
+
+// Two-kernel split-K: phase A writes one partial sum per (row split, column)
+// with plain stores; phase B (next launch, so the kernel boundary orders the
+// stores) adds the partials in split order and applies the epilogue.
+template <int R, int CPW, int U>
+__global__ __launch_bounds__(256) void matvec_partial_kernel(MatvecArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i_base = blockIdx.x * (MV_WAVES * CPW) + wave * CPW;
+    const int s = blockIdx.y;
+    const int rows_per = (((a.N + a.split - 1) / a.split) + 63) & ~63;
+    int j_begin = s * rows_per;
+    int j_end = min(a.N, j_begin + rows_per);
+    if (a.tri == TRI_LOWER) j_begin = max(j_begin, (i_base / 64) * 64);
+    if (a.tri == TRI_UPPER) j_end = min(j_end, i_base + CPW);
+    double yr[CPW][R], yi[CPW][R];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            yr[c][r] = 0.0;
+            yi[c][r] = 0.0;
+        }
+    const long ld = a.ld;
+    int ic[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) ic[c] = min(i_base + c, a.N - 1);
+    for (int j0 = j_begin; j0 < j_end; j0 += 64 * U) {
+        double m[U][CPW];
+        cplx xv[U][R];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * 64 + lane;
+            const bool ok = j < j_end;
+            const int jj = ok ? j : j_begin;
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const double v = a.M[jj + ld * ic[c]];
+                const int i = i_base + c;
+                bool keep = ok && i < a.N;
+                if (a.tri == TRI_LOWER) keep = keep && (jj >= i);
+                if (a.tri == TRI_UPPER) keep = keep && (jj <= i);
+                m[u][c] = keep ? v : 0.0;
+            }
+            const cplx* xp = a.x + (long)jj * a.x_stride + a.x_off;
+#pragma unroll
+            for (int r = 0; r < R; ++r) xv[u][r] = xp[r];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < CPW; ++c)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    yr[c][r] = fma(m[u][c], xv[u][r].re, yr[c][r]);
+                    yi[c][r] = fma(m[u][c], xv[u][r].im, yi[c][r]);
+                }
+    }
+    double my_re = 0.0, my_im = 0.0;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double sr = wave_sum(yr[c][r]);
+            const double si = wave_sum(yi[c][r]);
+            if (lane == c * R + r) {
+                my_re = sr;
+                my_im = si;
+            }
+        }
+    if (lane >= CPW * R) return;
+    const int c = lane / R, r = lane % R;
+    const int i = i_base + c;
+    if (i < a.N) a.partial[((long)s * a.N + i) * R + r] = cplx{my_re, my_im};
+}
+
+__global__ void matvec_reduce_epilogue_kernel(MatvecArgs a) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)a.N * a.R) return;
+    const int i = (int)(e / a.R), r = (int)(e % a.R);
+    cplx y{0.0, 0.0};
+    for (int s = 0; s < a.split; ++s) {
+        const cplx p = a.partial[((long)s * a.N + i) * a.R + r];
+        y.re += p.re;
+        y.im += p.im;
+    }
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    if (a.epi == EPI_PLAIN) {
+        a.out[oi] = y;
+        return;
+    }
+    const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+    cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+    if (a.epi == EPI_CHEB) {
+        const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+        T.re -= x2.re;
+        T.im -= x2.im;
+        const cplx add = cmul(a.coef[a.k], T);
+        cplx acc = a.acc[oi];
+        acc.re += add.re;
+        acc.im += add.im;
+        a.acc[oi] = acc;
+    } else {
+        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+    a.out[oi] = T;
+}
+
 int matvec_split_for(int N, int R) {
     const int cpw = R <= 4 ? 4 : (R <= 8 ? 2 : 1);
     const int colblocks = (N + MV_WAVES * cpw - 1) / (MV_WAVES * cpw);
@@ -177,36 +457,109 @@ int matvec_split_for(int N, int R) {
 }
 
 template <int R, int CPW>
-static void launch_mv(const MatvecArgs& a, hipStream_t st) {
+static void launch_mv_split(const MatvecArgs& a, hipStream_t st) {
     const int colblocks = (a.N + MV_WAVES * CPW - 1) / (MV_WAVES * CPW);
     hipLaunchKernelGGL((matvec_kernel<R, CPW>), dim3(colblocks, a.split), dim3(256), 0, st, a);
 }
 
-void launch_matvec(const MatvecArgs& a, hipStream_t st) {
+template <int R, int CPW, int U>
+static void launch_mv_twophase(const MatvecArgs& a, int split, hipStream_t st) {
+    MatvecArgs b = a;
+    b.split = split;
+    const int colblocks = (a.N + MV_WAVES * CPW - 1) / (MV_WAVES * CPW);
+    hipLaunchKernelGGL((matvec_partial_kernel<R, CPW, U>), dim3(colblocks, split), dim3(256), 0, st, b);
+    const long n = (long)a.N * a.R;
+    hipLaunchKernelGGL(matvec_reduce_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b);
+}
+
+template <int R, int CPW, int NW, int JC, int PIPE = 0>
+static void launch_mv_cols(const MatvecArgs& a, hipStream_t st) {
+    const int nc = NW * CPW;
+    const int blocks = (a.N + nc - 1) / nc;
+    hipLaunchKernelGGL((matvec_cols_kernel<R, CPW, NW, JC, PIPE>), dim3(blocks), dim3(NW * 64), 0, st, a);
+}
+
+// Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
+// counters); 1..4 = column-block kernels of different shapes.
+constexpr int MV_VARIANTS = 15;
+
+template <int R>
+static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
+    // LDS tile height: a multiple of 64 rows, 64 KB of double-buffered x at most
+    constexpr int JC = ((2048 / R) > 512 ? 512 : (2048 / R)) & ~63;
+    constexpr int CS = R <= 4 ? 4 : (R <= 8 ? 2 : 1);
+    switch (variant) {
+        case 0: launch_mv_split<R, CS>(a, st); break;
+        case 1: launch_mv_cols<R, 2, 8, JC>(a, st); break;
+        case 2: launch_mv_cols<R, 4, 4, JC>(a, st); break;
+        case 3: launch_mv_cols<R, 1, 8, JC>(a, st); break;
+        case 4: launch_mv_cols<R, 2, 4, JC>(a, st); break;
+        // two-phase split-K; the split keeps split * N * R within the partial buffer
+        case 5: launch_mv_twophase<R, CS, 2>(a, a.split, st); break;
+        case 6: launch_mv_twophase<R, CS, 4>(a, a.split, st); break;
+        case 7: launch_mv_twophase<R, CS, 2>(a, a.split > 1 ? (a.split + 1) / 2 : 1, st); break;
+        case 8: launch_mv_twophase<R, (CS > 1 ? CS / 2 : 1), 2>(a, a.split, st); break;
+        // column-block kernels with a row-by-row LDS schedule (lower register use)
+        case 9: launch_mv_cols<R, 2, 8, JC, 1>(a, st); break;
+        case 10: launch_mv_cols<R, 2, 8, JC, 2>(a, st); break;
+        case 11: launch_mv_cols<R, 1, 16, JC, 1>(a, st); break;
+        case 12: launch_mv_cols<R, 2, 8, (JC >= 512 ? 256 : JC), 1>(a, st); break;
+        case 13: launch_mv_cols<R, 4, 4, JC, 1>(a, st); break;
+        default: launch_mv_cols<R, 1, 8, JC, 1>(a, st); break;
+    }
+}
+
+static int g_variant_override = -1;
+struct TuneKey {
+    int N, R, tri;
+};
+static std::vector<std::pair<TuneKey, int>> g_tuned;
+
+int matvec_variant_for(int N, int R, int tri) {
+    if (g_variant_override >= 0) return g_variant_override;
+    for (auto& kv : g_tuned)
+        if (kv.first.N == N && kv.first.R == R && kv.first.tri == tri) return kv.second;
+    return 1;
+}
+void matvec_set_tuned(int N, int R, int tri, int variant) {
+    for (auto& kv : g_tuned)
+        if (kv.first.N == N && kv.first.R == R && kv.first.tri == tri) {
+            kv.second = variant;
+            return;
+        }
+    g_tuned.push_back({TuneKey{N, R, tri}, variant});
+}
+void matvec_force_variant(int v) { g_variant_override = v; }
+int matvec_variant_count() { return MV_VARIANTS; }
+
+void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
     switch (a.R) {
-        case 1: launch_mv<1, 4>(a, st); break;
-        case 2: launch_mv<2, 4>(a, st); break;
-        case 3: launch_mv<3, 4>(a, st); break;
-        case 4: launch_mv<4, 4>(a, st); break;
-        case 6: launch_mv<6, 2>(a, st); break;
-        case 8: launch_mv<8, 2>(a, st); break;
-        case 12: launch_mv<12, 1>(a, st); break;
-        case 16: launch_mv<16, 1>(a, st); break;
+        case 1: launch_mv_r<1>(a, variant, st); break;
+        case 2: launch_mv_r<2>(a, variant, st); break;
+        case 3: launch_mv_r<3>(a, variant, st); break;
+        case 4: launch_mv_r<4>(a, variant, st); break;
+        case 6: launch_mv_r<6>(a, variant, st); break;
+        case 8: launch_mv_r<8>(a, variant, st); break;
+        case 12: launch_mv_r<12>(a, variant, st); break;
+        case 16: launch_mv_r<16>(a, variant, st); break;
         default: {
-            // generic: process in chunks of up to 16 columns
             int done = 0;
             while (done < a.R) {
-                int take = a.R - done;
-                int t = take >= 16 ? 16 : take >= 12 ? 12 : take >= 8 ? 8 : take >= 6 ? 6 : take >= 4 ? 4 : take >= 3 ? 3 : take >= 2 ? 2 : 1;
+                const int take = a.R - done;
+                const int t = take >= 16 ? 16 : take >= 12 ? 12 : take >= 8 ? 8 : take >= 6 ? 6 : take >= 4 ? 4 : take >= 3 ? 3 : take >= 2 ? 2 : 1;
                 MatvecArgs b = a;
                 b.R = t;
                 b.x_off = a.x_off + done;
                 b.out_off = a.out_off + done;
-                launch_matvec(b, st);
+                launch_matvec_variant(b, variant, st);
                 done += t;
             }
         }
     }
+}
+
+void launch_matvec(const MatvecArgs& a, hipStream_t st) {
+    launch_matvec_variant(a, matvec_variant_for(a.N, a.R, a.tri), st);
 }
 
 // ------------------------------------------------------------------ Lanczos

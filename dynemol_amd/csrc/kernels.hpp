@@ -79,6 +79,11 @@ struct MatvecArgs {
 };
 int matvec_split_for(int N, int R);
 void launch_matvec(const MatvecArgs& a, hipStream_t st);
+void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st);
+int matvec_variant_for(int N, int R, int tri);
+void matvec_set_tuned(int N, int R, int tri, int variant);
+void matvec_force_variant(int v);
+int matvec_variant_count();
 
 void launch_zero_u32(unsigned* p, int n, hipStream_t st);
 void launch_zero_cplx(cplx* p, long n, hipStream_t st);

@@ -31,9 +31,10 @@ SYMBOLS = [
     "dh_last_error", "dh_version", "dh_init", "dh_finalize", "dh_system_create", "dh_system_destroy",
     "dh_system_bytes", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
-    "dh_get_observables", "dh_get_stats", "dh_propagation_elhl",
+    "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_propagation_elhl",
 ]
-TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky"]
+TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
+                "dh_test_matvec_timing"]
 
 # Every live context and system is released before interpreter shutdown, while
 # the HIP runtime is still up (a handle freed during finalisation aborts).
@@ -85,6 +86,11 @@ def lib():
                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_int]
     L.dh_test_cholesky.argtypes = [ctypes.c_int, _d, _d, _i]
+    L.dh_test_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
+    L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
+    L.dh_matvec_tuning.argtypes = [_vp, _i]
+    L.dh_test_force_matvec_variant.argtypes = [ctypes.c_int]
+    L.dh_test_matvec_timing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
     _lib = L
     return L
 
@@ -239,10 +245,20 @@ class System:
         return ao, pops, complex(erg[0], erg[1])
 
     def stats(self):
-        out = np.zeros(5)
+        out = np.zeros(9)
         _check(lib().dh_get_stats(self.h, _p(out)), "dh_get_stats")
         return {"terms": int(out[0]), "steps": int(out[1]), "matvecs": int(out[2]), "retries": int(out[3]),
-                "norm_drift": float(out[4])}
+                "norm_drift": float(out[4]), "series_ms": float(out[5]), "series_launches": int(out[6]),
+                "rebuilds": int(out[7])}
+
+    def tuning(self):
+        out = np.zeros(7, dtype=np.int32)
+        _check(lib().dh_matvec_tuning(self.h, _p(out, _i)), "dh_matvec_tuning")
+        keys = ["term", "energy", "L_phi_bra", "LinvT_phi", "LT_ket", "Linv_bra", "lanczos"]
+        return dict(zip(keys, out.tolist()))
+
+    def set_timing(self, on=True):
+        _check(lib().dh_set_timing(self.h, 1 if on else 0), "dh_set_timing")
 
 
 def run_gemm(A, B, C, alpha=1.0, beta=0.0, transA=0, transB=0, kmode=0, lower_only=0):
@@ -265,6 +281,22 @@ def run_cholesky(S):
     _check(lib().dh_test_cholesky(n, A.ctypes.data_as(_d), X.ctypes.data_as(_d), ctypes.byref(info)),
            "dh_test_cholesky")
     return A, X, info.value
+
+
+def force_matvec_variant(v):
+    _check(lib().dh_test_force_matvec_variant(int(v)), "dh_test_force_matvec_variant")
+
+
+def time_matvec(N, R, variant, reps=20):
+    us = ctypes.c_double(0.0)
+    _check(lib().dh_test_matvec_timing(N, R, variant, reps, ctypes.byref(us)), "dh_test_matvec_timing")
+    return us.value
+
+
+def run_copy(nbytes, repeats=20):
+    ms = ctypes.c_double(0.0)
+    _check(lib().dh_test_copy(int(nbytes), repeats, ctypes.byref(ms)), "dh_test_copy")
+    return ms.value
 
 
 def system_bytes(n_orb, n_wp):

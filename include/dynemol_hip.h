@@ -100,8 +100,19 @@ int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
  * last (data_output.f:87-147); erg (2) = QuasiParticleEnergies
  * sum_ij conj(ao_bra)_i h_ij Psi_ket_j (ElHl_Chebyshev.f:329-371). */
 int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, double *erg);
-/* Counters: [terms of last step, steps, matvecs, divergence retries, last norm drift]. */
-int dh_get_stats(dh_system *sys, double *out5);
+/* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
+ * retries, last norm drift, term-series GPU ms (HIP events, when timing is
+ * on), term launches timed, operator rebuilds, reserved]. */
+int dh_get_stats(dh_system *sys, double *out9);
+/* Record HIP events around each step's Chebyshev term series (on the
+ * context stream that runs them); no counterpart in the reference, whose
+ * only timing hook is the GPU_TIMING wall-clock print (GPU_Interface.cpp:33-49). */
+int dh_set_timing(dh_system *sys, int on);
+/* Matvec kernel variant chosen by the autotuner for this system's shapes:
+ * [Chebyshev term (2*n_wp RHS), <H> (n_wp), L phi_bra, L^-T phi, L^T ket,
+ * L^-1 bra, Lanczos (1 RHS)].  No counterpart in the reference (its GEMV tile
+ * sizes are compile-time macros, dzgemv_kernels.cu:581-608). */
+int dh_matvec_tuning(dh_system *sys, int *out7);
 
 /*
  * Drop-in for propagationelhl_gpucaller_ (Taylor_gpu.cpp:219-232, 634-736):

@@ -15,6 +15,14 @@ int dh_test_gemm(int M, int N, int K, const double *A, int lda, const double *B,
 /* In-place blocked Cholesky of an n x n SPD matrix (n a multiple of 64),
  * returning L (lower, upper zeroed) and L^-1 (lower) and the device info code. */
 int dh_test_cholesky(int n, double *A, double *Linv, int *info);
+/* A plain device copy of `bytes` (read + write), repeated; for calibrating
+ * memory-traffic counters against a kernel of known traffic. */
+int dh_test_copy(long long bytes, int repeats, double *ms_per_copy);
+/* Average launch time (microseconds, HIP events) of one matvec variant on an
+ * N x N FP64 operator with R complex right-hand sides (Chebyshev epilogue). */
+int dh_test_matvec_timing(int N, int R, int variant, int reps, double *us_per_launch);
+/* Force every matvec to one kernel variant (-1 restores the autotuned choice). */
+int dh_test_force_matvec_variant(int variant);
 #ifdef __cplusplus
 }
 #endif

@@ -80,3 +80,38 @@ def test_cholesky_reports_non_spd():
     S[70, 70] = -1.0
     _, _, info = hip.run_cholesky(S)
     assert info == 71
+
+
+@pytest.mark.parametrize("variant", list(range(15)))
+@pytest.mark.parametrize("n_wp", [1, 2, 3])
+def test_matvec_variants_propagate_exactly(variant, n_wp):
+    """Every matvec kernel variant the autotuner may pick gives the exact propagator."""
+    rng = np.random.default_rng(10 + n_wp)
+    n = 300
+    _say(f"variant {variant} n_wp {n_wp} N={n}")
+    X = rng.normal(size=(n, n))
+    S = X @ X.T / n + np.eye(n)
+    A = rng.normal(size=(n, n))
+    h = (A + A.T) * 2.0
+    import scipy.linalg as sl
+    E, C = sl.eigh(h, S)
+    dt = 5e-4
+    hip.force_matvec_variant(variant)
+    try:
+        ctx = hip.Context(0)
+        s = hip.System(ctx, None, n_wp=n_wp, n_frag=1, n_orb=n)
+        s.load_operators(S, h)
+        kets = [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(n_wp)]
+        bras = [S @ k for k in kets]
+        for w in range(n_wp):
+            s.set_wavepacket(w, bras[w], kets[w])
+        s.propagate(0.0, dt)
+        for w in range(n_wp):
+            b, k = s.wavepacket(w)
+            ke = C @ (np.exp(-1j * E * dt / hip.H_BAR) * (C.T @ (S @ kets[w])))
+            assert np.abs(k - ke).max() <= 1e-10 * np.abs(ke).max()
+            assert np.abs(b - S @ ke).max() <= 1e-10 * np.abs(S @ ke).max()
+        s.close()
+        ctx.close()
+    finally:
+        hip.force_matvec_variant(-1)
