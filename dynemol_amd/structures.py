@@ -232,15 +232,25 @@ def log_uniform_int(rng, lo, hi):
 
 # ----------------------------------------------------------------------------- PDB
 def write_pdb(path, s: Structure, title="dynemol-hip synthetic system"):
+    """Fixed columns of babel.f:290 (format 115) on HETATM records."""
+    def put(buf, col, text):  # col is 1-based as in the Fortran T edit descriptor
+        buf[col - 1:col - 1 + len(text)] = list(text)
+
     lines = [f"{title:<72.72}"]
     span = s.coords.max(axis=0) - s.coords.min(axis=0) + 10.0
     lines.append("CRYST1" + "".join(f"{v:9.3f}" for v in span) + "  90.00  90.00  90.00 P 1           1")
     for i, (sym, x) in enumerate(zip(s.symbols, s.coords)):
-        name = sym
         res = s.fragment_names[s.fragments[i]][:3] if s.fragment_names else "UNK"
-        line = ("HETATM" + f"{i + 1:5d}" + " " + f"{name:<5.5}" + f"{res:>3.3}" + "   "
-                + f"{1:7d}" + "   " + f"{x[0]:8.3f}{x[1]:8.3f}{x[2]:8.3f}" + "  1.00  0.00" + " " * 10 + f"{sym:>2}")
-        lines.append(line)
+        buf = [" "] * 80
+        put(buf, 1, "HETATM")
+        put(buf, 7, f"{(i + 1) % 100000:5d}")
+        put(buf, 12, f"{sym:<5.5}")
+        put(buf, 18, f"{res:<3.3}")
+        put(buf, 23, f"{1:7d}")
+        put(buf, 31, f"{x[0]:8.3f}{x[1]:8.3f}{x[2]:8.3f}")
+        put(buf, 55, "  1.00  0.00")
+        put(buf, 77, f"{sym:>2}")
+        lines.append("".join(buf).rstrip())
     lines.append("MASTER")
     lines.append("END")
     with open(path, "w") as f:

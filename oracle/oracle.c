@@ -42,6 +42,10 @@
 #define MXH (MXN + 3 * MXL)
 #define LT MXL /* ltot of the rotation arrays */
 
+/* Parallel regions only for large N: at small N thousands of tiny regions per
+ * propagation cost more than the work (and stall behind other spinning pools). */
+#define ORC_PAR_N 1024
+
 static double re_[1001], reali_[1001], fact_[151], facti_[151];
 static double root_[2 * MXL + 1], rooti_[2 * MXL + 1];
 static int tables_ready = 0;
@@ -537,7 +541,7 @@ int orc_cholesky(int n, const double *A, double *L) {
         if (!(d > 0.0)) return (int)(j + 1);
         d = sqrt(d);
         L[j + N * j] = d;
-#pragma omp parallel for schedule(static) if (N > 256)
+#pragma omp parallel for schedule(static) if (N >= ORC_PAR_N)
         for (long i = j + 1; i < N; ++i) {
             double s = L[i + N * j];
             for (long k = 0; k < j; ++k) s -= L[i + N * k] * L[j + N * k];
@@ -612,7 +616,7 @@ static const double H_BAR = 6.58264e-4; /* constants_m.f:23, eV*ps */
 
 /* res = alpha * H' * ket  (Matrix_math.f:291-301, dzgemv 'N') */
 static void op_x_ket(long N, zc *res, const double *H, const zc *ket, zc alpha) {
-#pragma omp parallel
+#pragma omp parallel if (N >= ORC_PAR_N)
     {
 #pragma omp for schedule(static)
         for (long i = 0; i < N; ++i) res[i] = 0.0;
@@ -635,7 +639,7 @@ static void op_x_ket(long N, zc *res, const double *H, const zc *ket, zc alpha) 
 
 /* res = alpha * H'^T * bra  (Matrix_math.f:238-248, dzgemv 'T') */
 static void bra_x_op(long N, zc *res, const zc *bra, const double *H, zc alpha) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N >= ORC_PAR_N)
     for (long j = 0; j < N; ++j) {
         const double *h = &H[N * j];
         double sr = 0.0, si = 0.0;
@@ -649,7 +653,7 @@ static void bra_x_op(long N, zc *res, const zc *bra, const double *H, zc alpha) 
 
 static double dotc_abs(long N, const zc *a, const zc *b) {
     double sr = 0.0, si = 0.0;
-#pragma omp parallel for reduction(+ : sr, si) schedule(static)
+#pragma omp parallel for reduction(+ : sr, si) schedule(static) if (N >= ORC_PAR_N)
     for (long i = 0; i < N; ++i) {
         zc p = conj(a[i]) * b[i];
         sr += creal(p);

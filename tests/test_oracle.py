@@ -1,0 +1,291 @@
+"""Pins the CPU oracle against closed forms, independent quadrature and invariants.
+
+The reference ships no tests or golden vectors for this path, so every check
+here is anchored on what the mathematics fixes:
+  * SOLAP lined-up overlaps == 2-D quadrature in prolate spheroidal coordinates
+    (sigma, pi, delta; both z_a >= z_b and z_a < z_b branches; Ti 3d double zeta);
+  * the 1s-1s closed form e^{-p}(1 + p + p^2/3);
+  * ROTAR/DLMN == a real-harmonic rotation built independently by projection on
+    a spherical quadrature grid, through molecular-frame S elements of p-d and
+    d-d pairs at general orientations;
+  * assembled S: symmetric, unit diagonal, |S_ij| <= 1, SPD, zero beyond the
+    cutoff, spectrum invariant under rigid rotation/translation;
+  * Hueckel h: the Wolfsberg-Helmholz closed form element by element;
+  * Taylor propagation: the exact (h, S) eigen-propagator, group property.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from numpy.polynomial.legendre import leggauss
+from scipy.special import lpmv
+
+from dynemol_amd.eht import ELEMENTS, build_basis
+from dynemol_amd.structures import Structure, ch_cluster
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------- SOLAP
+def lined_up(na, la, za, nb, lb, zb, R):
+    """Normalised lined-up overlaps <n l m | n' l' m>, m = 0..min(l,l'), as Build_Overlap_Matrix forms them."""
+    s = O.solap(na, la, za, nb, lb, zb, R)
+    anor = O.anor(za, zb, na, nb, la, lb)
+    msup = min(la, lb)
+    out = np.zeros(msup + 1)
+    for m in range(msup):
+        out[m] = s[m] * anor
+        anor = anor / O.anor_div(la, lb, m)
+        if m == 0:
+            anor = anor + anor
+    out[msup] = s[msup] * anor
+    return out
+
+
+def _nlm(l, m):
+    return math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - m) / math.factorial(l + m))
+
+
+def quad_lined_up(na, la, za, nb, lb, zb, R, m, nx=300, ne=160):
+    """<a|b> for normalised real STOs, A at the origin and B at +R z, by quadrature."""
+    u, wu = leggauss(nx)
+    u = (u + 1) / 2
+    wu = wu / 2
+    scale = 4.0 / (za + zb)
+    xi = 1 + u / (1 - u) * scale
+    dxi = wu * scale / (1 - u) ** 2
+    eta, we = leggauss(ne)
+    X, E = np.meshgrid(xi, eta, indexing="ij")
+    W = np.outer(dxi, we)
+    ra, rb = R * (X + E) / 2, R * (X - E) / 2
+    ca, cb = (1 + X * E) / (X + E), (X * E - 1) / (X - E)
+    Na = (2 * za) ** (na + 0.5) / math.sqrt(math.factorial(2 * na))
+    Nb = (2 * zb) ** (nb + 0.5) / math.sqrt(math.factorial(2 * nb))
+    fa = Na * ra ** (na - 1) * np.exp(-za * ra) * _nlm(la, m) * lpmv(m, la, ca)
+    fb = Nb * rb ** (nb - 1) * np.exp(-zb * rb) * _nlm(lb, m) * lpmv(m, lb, cb)
+    return 2 * math.pi * np.sum(W * (R / 2) ** 3 * (X * X - E * E) * fa * fb)
+
+
+SOLAP_CASES = [
+    (2, 1, 3.0708, 2, 1, 3.0708, 1.40),   # C 2p - C 2p
+    (2, 0, 3.0708, 2, 1, 3.6849, 1.50),   # C 2s - N 2p (z_a < z_b)
+    (2, 1, 4.2991, 2, 0, 3.0708, 1.45),   # O 2p - C 2s (z_a > z_b)
+    (1, 0, 2.4566, 2, 1, 3.0708, 1.09),   # H 1s - C 2p
+    (3, 2, 8.5982, 2, 1, 4.2991, 2.00),   # Ti 3d (tight zeta) - O 2p
+    (3, 2, 2.6456, 3, 2, 2.6456, 2.80),   # Ti 3d - Ti 3d (diffuse zeta): sigma, pi, delta
+    (4, 1, 2.0314, 3, 2, 2.6456, 2.50),   # Ti 4p - Ti 3d
+    (4, 0, 2.0314, 2, 1, 4.2991, 1.95),   # Ti 4s - O 2p
+    (3, 2, 2.6456, 4, 1, 2.0314, 3.00),   # Ti 3d - Ti 4p
+    (2, 1, 3.0708, 2, 1, 3.0708, 6.50),   # long range
+]
+
+
+@pytest.mark.parametrize("case", SOLAP_CASES)
+def test_solap_matches_quadrature(case):
+    o = lined_up(*case)
+    q = np.array([quad_lined_up(*case, m) for m in range(len(o))])
+    assert np.abs(o - q).max() < 1e-12, (o, q)
+
+
+def test_solap_1s1s_closed_form():
+    for z, R in [(2.4566, 0.74), (2.4566, 1.5), (1.3, 3.0)]:
+        p = z * R
+        assert abs(lined_up(1, 0, z, 1, 0, z, R)[0] - math.exp(-p) * (1 + p + p * p / 3)) < 1e-15
+
+
+# ---------------------------------------------------------------- rotations
+def real_harmonics(l, v):
+    """Real spherical harmonics (no Condon-Shortley phase) in the reference order m = -l..l.
+
+    l = 1: (y, z, x); l = 2: (xy, yz, z2, xz, x2-y2) (structure.f:211-223).
+    v: (..., 3) unit vectors.
+    """
+    x, y, z = v[..., 0], v[..., 1], v[..., 2]
+    if l == 0:
+        return np.ones(v.shape[:-1] + (1,)) / math.sqrt(4 * math.pi)
+    if l == 1:
+        c = math.sqrt(3 / (4 * math.pi))
+        return c * np.stack([y, z, x], -1)
+    c = math.sqrt(15 / (4 * math.pi))
+    return np.stack([c * x * y, c * y * z, math.sqrt(5 / (16 * math.pi)) * (3 * z * z - 1), c * x * z,
+                     c / 2 * (x * x - y * y)], -1)
+
+
+def sphere_grid(n=24):
+    t, wt = leggauss(n)
+    ph = np.linspace(0, 2 * math.pi, 2 * n, endpoint=False)
+    T, P = np.meshgrid(t, ph, indexing="ij")
+    st = np.sqrt(1 - T * T)
+    v = np.stack([st * np.cos(P), st * np.sin(P), T], -1).reshape(-1, 3)
+    w = (np.outer(wt, np.full(2 * n, 2 * math.pi / (2 * n)))).reshape(-1)
+    return v, w
+
+
+def rotation_to(direction):
+    """A rotation matrix whose third column is the unit vector `direction`."""
+    d = direction / np.linalg.norm(direction)
+    a = np.array([1.0, 0, 0]) if abs(d[0]) < 0.9 else np.array([0, 1.0, 0])
+    e1 = np.cross(a, d)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(d, e1)
+    return np.stack([e1, e2, d], 1)
+
+
+def harmonic_rotation(l, Rm):
+    """D[m', m] = <Y_m' | Y_m o R^T>: molecular-frame coefficient of a lined-up real harmonic."""
+    v, w = sphere_grid()
+    Ymol = real_harmonics(l, v)
+    Yline = real_harmonics(l, v @ Rm)  # Y_m in the lined-up frame evaluated at molecular points
+    return np.einsum("pa,pb,p->ab", Ymol, Yline, w)
+
+
+def test_rotar_first_column_and_orthogonality():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        R, rl = O.rotation(np.zeros(3), d * 2.0, 2)
+        assert abs(R - 2.0) < 1e-14
+        for l in (1, 2):
+            D = rl[l]
+            assert np.abs(D @ D.T - np.eye(2 * l + 1)).max() < 1e-13
+            # lined-up m = 0 function seen in the molecular frame: Y_m'(d) / Y_0(z)
+            col = real_harmonics(l, d[None, :])[0] / real_harmonics(l, np.array([[0, 0, 1.0]]))[0][l]
+            assert np.abs(D[:, l] - col).max() < 1e-13
+
+
+def _two_atom(sym_a, sym_b, xa, xb):
+    st = Structure([sym_a, sym_b], np.array([xa, xb]), np.array([0, 0], dtype=np.int32))
+    return st, st.basis()
+
+
+@pytest.mark.parametrize("pair", [("O", "Ti"), ("Ti", "Ti"), ("C", "Ti"), ("H", "Ti"), ("C", "N")])
+def test_molecular_frame_S_matches_independent_rotation(pair):
+    """S_ab for general orientations == lined-up quadrature values rotated by an independent D."""
+    rng = np.random.default_rng(sum(map(ord, "".join(pair))))
+    for _ in range(3):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        Rab = rng.uniform(1.8, 3.2)
+        xa = rng.normal(size=3)
+        st, b = _two_atom(pair[0], pair[1], xa + d * Rab, xa)  # atom 0 = B (lower index), atom 1 = A
+        S = O.overlap(st.coords, b)
+        # reference convention: A = higher atom index, B = lower; lined-up z along A -> B
+        dirAB = (st.coords[0] - st.coords[1]) / Rab
+        Rm = rotation_to(dirAB)
+        ea, eb = ELEMENTS[pair[1]], ELEMENTS[pair[0]]
+        for sha in ea.shells:
+            for shb in eb.shells:
+                la, lb = sha.l, shb.l
+                lu = np.zeros(min(la, lb) + 1)
+                for i in range(sha.nzeta):
+                    for j in range(shb.nzeta):
+                        lu += sha.coef[i] * shb.coef[j] * np.array(
+                            [quad_lined_up(sha.n, la, sha.zeta[i], shb.n, lb, shb.zeta[j], Rab, m, 240, 120)
+                             for m in range(min(la, lb) + 1)])
+                Da, Db = harmonic_rotation(la, Rm), harmonic_rotation(lb, Rm)
+                oa = b.first_orb[1] + sum(2 * s.l + 1 for s in ea.shells if s.l < la)
+                ob = b.first_orb[0] + sum(2 * s.l + 1 for s in eb.shells if s.l < lb)
+                for ma in range(-la, la + 1):
+                    for mb in range(-lb, lb + 1):
+                        val = lu[0] * Da[ma + la, la] * Db[mb + lb, lb]
+                        for k in range(1, min(la, lb) + 1):
+                            val += lu[k] * (Da[ma + la, la + k] * Db[mb + lb, lb + k] +
+                                            Da[ma + la, la - k] * Db[mb + lb, lb - k])
+                        assert abs(S[oa + ma + la, ob + mb + lb] - val) < 1e-11
+
+
+# ---------------------------------------------------------------- assembled S and h
+@pytest.fixture(scope="module")
+def c1():
+    st = ch_cluster(seed=1)
+    b = st.basis()
+    S = O.overlap(st.coords, b)
+    h = O.huckel(b, S)
+    return st, b, S, h
+
+
+def test_S_invariants(c1):
+    st, b, S, h = c1
+    assert np.array_equal(S, S.T)
+    assert np.array_equal(np.diag(S), np.ones(b.N))
+    assert np.abs(S - np.eye(b.N)).max() <= 1.0
+    w = np.linalg.eigvalsh(S)
+    assert w.min() > 0
+    # same-atom off-diagonal blocks are zero
+    for a in range(st.n_atoms):
+        blk = S[b.first_orb[a]:b.first_orb[a + 1], b.first_orb[a]:b.first_orb[a + 1]]
+        assert np.array_equal(blk, np.eye(blk.shape[0]))
+    # rigid rotation + translation leaves the spectrum unchanged
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    S2 = O.overlap(st.coords @ Q.T + rng.normal(size=3) * 5, b)
+    assert np.abs(np.linalg.eigvalsh(S2) - w).max() < 1e-12
+
+
+def test_S_cutoff():
+    st = Structure(["C", "C"], np.array([[0, 0, 0], [12.5, 0, 0.0]]), np.zeros(2, dtype=np.int32))
+    b = st.basis()
+    assert np.array_equal(O.overlap(st.coords, b), np.eye(8))
+    st.coords[1, 0] = 11.5
+    S = O.overlap(st.coords, b)
+    assert np.abs(S - np.eye(8)).max() > 0
+
+
+def test_huckel_closed_form(c1):
+    st, b, S, h = c1
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        i, j = rng.integers(0, b.N, 2)
+        if i == j:
+            ref = b.ip[i] + b.v_shift[i]
+        else:
+            d = (b.ip[i] - b.ip[j]) / (b.ip[i] + b.ip[j])
+            k = 0.5 * (b.k_wh[i] + b.k_wh[j])
+            keff = k + d * d + d ** 4 * (1 - k)
+            ref = keff * 0.5 * (b.ip[i] + b.ip[j]) + 0.5 * (b.v_shift[i] + b.v_shift[j])
+        assert abs(h[i, j] - ref * S[i, j]) < 1e-13 * max(1.0, abs(ref))
+
+
+# ---------------------------------------------------------------- propagation
+def test_taylor_vs_exact_propagator(c1):
+    st, b, S, h = c1
+    E, C = sl.eigh(h, S)
+    Hp = O.dgemm(O.syinvert(S), h)
+    assert np.abs(Hp - np.linalg.solve(S, h)).max() < 1e-11
+    dt = 5e-4
+    tau = dt / O.H_BAR
+    ket = C[:, 208] + 0j
+    exact = C @ (np.exp(-1j * E * dt / O.H_BAR) * (C.T @ (S @ ket)))
+    b1, k1, _, st1, stats = O.propagation(Hp, S @ ket, ket, 0.0, dt, tau)  # reference tolerance 1e-8
+    assert np.abs(k1 - exact).max() < 5e-8
+    b2, k2, _, _, _ = O.propagation(Hp, S @ ket, ket, 0.0, dt, tau, 1e-13, 1e-11)
+    assert np.abs(k2 - exact).max() < 1e-11
+    assert np.abs(b2 - S @ exact).max() < 1e-11
+    # a stationary state only gains the phase e^{-iE t/hbar}
+    assert np.abs(k2 - np.exp(-1j * E[208] * dt / O.H_BAR) * ket).max() < 1e-11
+
+
+def test_taylor_group_property(c1):
+    st, b, S, h = c1
+    Hp = np.linalg.solve(S, h)
+    E, C = sl.eigh(h, S)
+    rng = np.random.default_rng(5)
+    x = C[:, 196:220] @ (rng.normal(size=24) + 1j * rng.normal(size=24))  # frontier superposition
+    tau = 5e-4 / O.H_BAR
+    _, k_two, _, _, _ = O.propagation(Hp, S @ x, x, 0.0, 2.5e-4, tau, 1e-13, 1e-11)
+    _, k_two, _, _, _ = O.propagation(Hp, S @ k_two, k_two, 2.5e-4, 5e-4, tau, 1e-13, 1e-11)
+    _, k_one, _, _, _ = O.propagation(Hp, S @ x, x, 0.0, 5e-4, tau, 1e-13, 1e-11)
+    assert np.abs(k_two - k_one).max() < 1e-10 * np.abs(k_one).max()
+
+
+def test_populations_and_energy_invariants(c1):
+    st, b, S, h = c1
+    rng = np.random.default_rng(6)
+    ket = rng.normal(size=b.N) + 1j * rng.normal(size=b.N)
+    bra = S @ ket
+    pops = O.populations(b.fragment, 2, np.conj(ket), bra)
+    assert abs(pops[0] + pops[1] - pops[2]) < 1e-10
+    assert abs(pops[2] - np.real(np.conj(ket) @ S @ ket)) < 1e-9
+    e = O.qp_energy(np.conj(np.linalg.solve(S, bra)), h, ket)
+    assert abs(e - np.conj(ket) @ h @ ket) < 1e-8 * abs(e)
