@@ -61,7 +61,7 @@ struct dh_system {
     cplx* coef = nullptr;
     int* kout = nullptr;
     // observables
-    double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr;
+    double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr, *norms1 = nullptr;
     // host state
     bool have_coords = false, ops_ready = false, phi_dirty = true, state_set = false, hprime_ready = false;
     double bounds_h[6] = {0, 0, 0, 0, 0, 0};
@@ -77,6 +77,9 @@ struct dh_system {
     double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool timing = false;
     bool tuned = false;
+    int precision = 0;        // 0 = FP64, 1 = mixed (FP32 MFMA term GEMM)
+    float* H32 = nullptr;     // FP32 copy of H_orth for the mixed mode
+    bool h32_ready = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -154,11 +157,12 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->ctx->ws_owner == s) s->ctx->ws_owner = nullptr;
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
+    if (s->H32) hipFree(s->H32);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->bounds, s->J, s->coef,
-                    s->kout, s->pops, s->erg, s->norms, s->norms0};
+                    s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete s;
@@ -331,6 +335,7 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->erg, (size_t)n_wp * 2);
     SALLOC(s->norms, (size_t)n_wp * 2);
     SALLOC(s->norms0, (size_t)R);
+    SALLOC(s->norms1, (size_t)R);
     DH_HIP(hipEventCreate(&s->ev0));
     DH_HIP(hipEventCreate(&s->ev1));
     launch_zero_u32(s->counters, ld, st);
@@ -536,6 +541,11 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
         int trc = autotune_system(s);
         if (trc) return trc;
     }
+    s->h32_ready = false;
+    if (s->precision == 1) {
+        launch_to_f32(s->Hth, s->H32, (long)ld * ld, st);
+        s->h32_ready = true;
+    }
     int rc = compute_bounds(s);
     if (rc) return rc;
     s->ops_ready = true;
@@ -683,6 +693,12 @@ static int chebyshev_series(dh_system* s) {
     a.out_stride = R; a.out_off = 0;
     a.acc = s->acc;
     a.coef = s->coef;
+    a.M32 = s->H32;
+    const bool mixed = s->precision == 1;
+    auto term = [&](const MatvecArgs& args) {
+        if (mixed) launch_cheb_gemm(args, 1, st);
+        else launch_matvec(args, st);
+    };
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
     a.epi = EPI_CHEB_FIRST;
     a.x = s->phi;
@@ -690,7 +706,7 @@ static int chebyshev_series(dh_system* s) {
     a.alpha = 1.0 / r;
     a.beta = -c / r;
     a.k = 1;
-    launch_matvec(a, st);
+    term(a);
     cplx* prev2 = s->phi;
     cplx* prev = s->x1;
     for (int k = 2; k <= s->K; ++k) {
@@ -702,7 +718,7 @@ static int chebyshev_series(dh_system* s) {
         a.alpha = 2.0 / r;
         a.beta = -2.0 * c / r;
         a.k = k;
-        launch_matvec(a, st);
+        term(a);
         prev2 = prev;
         prev = outb;
     }
@@ -762,7 +778,10 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
         return DH_OK;
     }
     launch_column_norms(s->N, R, s->phi, s->norms0, st);
-    std::vector<double> n0(R), n1(2 * s->W);
+    std::vector<double> n0(R), n1(R);
+    // FP64 norm drift of one step that signals bounds not enclosing the
+    // spectrum; the mixed mode's FP32 products drift ~1e-7 per step by themselves
+    const double drift_tol = s->precision == 1 ? 1e-4 : 1e-9;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int rc = ensure_coefficients(s, dt);
         if (rc) return rc;
@@ -771,18 +790,14 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
         if (rc) return rc;
         if (s->timing) DH_HIP(hipEventRecord(s->ev1, st));
         std::swap(s->phi, s->acc);  // phi <- new state; acc keeps the old one
-        launch_outputs(s);
+        launch_column_norms(s->N, R, s->phi, s->norms1, st);
         DH_HIP(hipMemcpyAsync(n0.data(), s->norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
-        DH_HIP(hipMemcpyAsync(n1.data(), s->norms, sizeof(double) * 2 * s->W, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(n1.data(), s->norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         DH_HIP(hipStreamSynchronize(st));
-        // n0 holds |phi_c|^2 per column (bra columns 0..W-1, ket columns W..2W-1)
         double drift = 0.0;
-        for (int w = 0; w < s->W; ++w) {
-            const double b0 = n0[w], k0 = n0[s->W + w];
-            const double b1 = n1[2 * w], k1 = n1[2 * w + 1];
-            if (b0 > 0) drift = std::max(drift, std::fabs(b1 - b0) / b0);
-            if (k0 > 0) drift = std::max(drift, std::fabs(k1 - k0) / k0);
-            if (!std::isfinite(b1) || !std::isfinite(k1)) drift = INFINITY;
+        for (int c = 0; c < R; ++c) {
+            if (n0[c] > 0) drift = std::max(drift, std::fabs(n1[c] - n0[c]) / n0[c]);
+            if (!std::isfinite(n1[c])) drift = INFINITY;
         }
         s->stats[4] = drift;
         if (s->timing) {
@@ -791,7 +806,10 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
             s->stats[5] += ms;
             s->stats[6] += s->K;
         }
-        if (drift < 1e-9) {
+        if (drift < drift_tol) {
+            if (s->precision == 1) launch_renorm_columns(s->phi, s->N, R, s->norms0, s->norms1, st);
+            launch_outputs(s);
+            DH_HIP(hipStreamSynchronize(st));
             s->stats[0] = s->K + 1;
             s->stats[1] += 1;
             s->stats[2] += s->K + 3;
@@ -962,7 +980,7 @@ extern "C" int dh_matvec_tuning(dh_system* s, int* out7) {
 }
 
 extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double* us_per_launch) {
-    CHECK_ARG(N > 0 && R >= 1 && R <= 16 && variant >= 0 && variant < matvec_variant_count() && reps > 0,
+    CHECK_ARG(N > 0 && R >= 1 && R <= 128 && variant >= -1 && variant < matvec_variant_count() && reps > 0,
               "dh_test_matvec_timing: bad argument");
     const int ld = round_ld(N);
     double* M = nullptr;
@@ -990,9 +1008,19 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     hipEvent_t e0, e1;
     DH_HIP(hipEventCreate(&e0));
     DH_HIP(hipEventCreate(&e1));
-    for (int w = 0; w < 3; ++w) launch_matvec_variant(a, variant, nullptr);
+    float* M32 = nullptr;
+    if (variant == -1) {  // FP32 matrix-core GEMM (mixed precision)
+        DALLOC(M32, (size_t)ld * ld);
+        DH_HIP(hipMemset(M32, 0, sizeof(float) * (size_t)ld * ld));
+        a.M32 = M32;
+    }
+    auto run = [&]() {
+        if (variant == -1) launch_cheb_gemm(a, 1, nullptr);
+        else launch_matvec_variant(a, variant, nullptr);
+    };
+    for (int w = 0; w < 3; ++w) run();
     DH_HIP(hipEventRecord(e0, nullptr));
-    for (int r = 0; r < reps; ++r) launch_matvec_variant(a, variant, nullptr);
+    for (int r = 0; r < reps; ++r) run();
     DH_HIP(hipEventRecord(e1, nullptr));
     DH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1000,5 +1028,260 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     *us_per_launch = 1e3 * ms / reps;
     hipEventDestroy(e0); hipEventDestroy(e1);
     hipFree(M); hipFree(x); hipFree(x2); hipFree(out); hipFree(acc); hipFree(coef); hipFree(partial); hipFree(counters);
+    if (M32) hipFree(M32);
+    return DH_OK;
+}
+
+extern "C" int dh_set_precision(dh_system* s, int mode) {
+    CHECK_ARG(s && (mode == 0 || mode == 1), "dh_set_precision: mode must be 0 (FP64) or 1 (mixed)");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    if (mode == 1 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
+    s->precision = mode;
+    if (mode == 1 && s->ops_ready && !s->h32_ready) {
+        launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);
+        DH_HIP(hipStreamSynchronize(s->ctx->stream));
+        s->h32_ready = true;
+    }
+    return DH_OK;
+}
+
+// ------------------------------------------------------------------ multi-system batches
+// Many systems (uneven N, same wavepacket count) advance together: one grouped
+// launch per Chebyshev term covers every system that still has terms left
+// (systems sorted by term count, so the active set is a prefix of the task
+// table), then grouped norm checks, commits and observables.
+struct dh_batch {
+    dh_context* ctx = nullptr;
+    std::vector<dh_system*> sys;
+    int W = 0, R = 0, n = 0;
+    enum { D_K1 = 0, D_K2, D_KODD, D_KEVEN, D_ENERGY, D_LBRA, D_LINVT, NDESC };
+    MatvecArgs* d_args = nullptr;  // [NDESC][n]
+    int *d_task_desc = nullptr, *d_task_blk = nullptr;
+    NormTask* d_norms = nullptr;   // [2][n*R]: before, after
+    CopyTask* d_copy = nullptr;    // [n]
+    ObsTask* d_obs = nullptr;      // [n*W]
+    double* d_nval = nullptr;      // [2][n*R]
+    std::vector<int> order, prefix;  // systems by K descending; prefix task counts
+    std::vector<int> Ks;
+    int Kmax = 0;
+    double stats[4] = {0, 0, 0, 0};  // steps, term launches, last max drift, retries
+};
+
+extern "C" int dh_batch_destroy(dh_batch* b) {
+    if (!b) return DH_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    void* ptrs[] = {b->d_args, b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs, b->d_nval};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete b;
+    return DH_OK;
+}
+
+extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems, dh_batch** out) {
+    CHECK_ARG(ctx && out && systems && n > 0, "dh_batch_create: bad argument");
+    const int W = systems[0]->W;
+    for (int i = 0; i < n; ++i) {
+        CHECK_ARG(systems[i] && systems[i]->ctx == ctx, "dh_batch_create: systems must belong to the context");
+        CHECK_ARG(systems[i]->W == W, "dh_batch_create: all systems need the same wavepacket count");
+    }
+    const int R = 2 * W;
+    CHECK_ARG(R == 2 || R == 4 || R == 8 || R == 16, "dh_batch_create: wavepackets per system must be 1, 2, 4 or 8");
+    DH_HIP(hipSetDevice(ctx->device));
+    dh_batch* b = new dh_batch();
+    b->ctx = ctx;
+    b->sys.assign(systems, systems + n);
+    b->W = W;
+    b->R = R;
+    b->n = n;
+    long tasks = 0;
+    for (auto* s : b->sys) tasks += (s->N + grouped_cols_per_block(R) - 1) / grouped_cols_per_block(R);
+    int rc;
+#define BALLOC(p, c)                                    \
+    do {                                                \
+        if ((rc = dalloc(&(p), (c))) != DH_OK) {        \
+            dh_batch_destroy(b);                        \
+            return rc;                                  \
+        }                                               \
+    } while (0)
+    BALLOC(b->d_args, (size_t)dh_batch::NDESC * n);
+    BALLOC(b->d_task_desc, (size_t)tasks);
+    BALLOC(b->d_task_blk, (size_t)tasks);
+    BALLOC(b->d_norms, (size_t)2 * n * R);
+    BALLOC(b->d_copy, (size_t)n);
+    BALLOC(b->d_obs, (size_t)n * W);
+    BALLOC(b->d_nval, (size_t)2 * n * R);
+#undef BALLOC
+    *out = b;
+    return DH_OK;
+}
+
+static MatvecArgs batch_desc(dh_system* s, int kind) {
+    const int W = s->W, R = 2 * W;
+    MatvecArgs a{};
+    a.ld = s->ld;
+    a.N = s->N;
+    a.coef = s->coef;
+    a.acc = s->acc;
+    const double c = s->coef_c, r = s->coef_r;
+    switch (kind) {
+        case dh_batch::D_K1:
+            a.M = s->Hth; a.R = R; a.tri = TRI_FULL; a.epi = EPI_CHEB_FIRST;
+            a.x = s->phi; a.out = s->x1; a.alpha = 1.0 / r; a.beta = -c / r;
+            a.x_stride = R; a.out_stride = R;
+            break;
+        case dh_batch::D_K2:
+            a.M = s->Hth; a.R = R; a.tri = TRI_FULL; a.epi = EPI_CHEB;
+            a.x = s->x1; a.x2 = s->phi; a.out = s->x0; a.alpha = 2.0 / r; a.beta = -2.0 * c / r;
+            a.x_stride = R; a.out_stride = R;
+            break;
+        case dh_batch::D_KODD:  // k = 3, 5, ...: T_{k-1} in x0, T_{k-2} in x1, T_k over x1
+            a.M = s->Hth; a.R = R; a.tri = TRI_FULL; a.epi = EPI_CHEB;
+            a.x = s->x0; a.x2 = s->x1; a.out = s->x1; a.alpha = 2.0 / r; a.beta = -2.0 * c / r;
+            a.x_stride = R; a.out_stride = R;
+            break;
+        case dh_batch::D_KEVEN:  // k = 4, 6, ...
+            a.M = s->Hth; a.R = R; a.tri = TRI_FULL; a.epi = EPI_CHEB;
+            a.x = s->x1; a.x2 = s->x0; a.out = s->x0; a.alpha = 2.0 / r; a.beta = -2.0 * c / r;
+            a.x_stride = R; a.out_stride = R;
+            break;
+        case dh_batch::D_ENERGY:
+            a.M = s->Hth; a.R = W; a.tri = TRI_FULL; a.epi = EPI_PLAIN;
+            a.x = s->phi; a.x_stride = R; a.x_off = W; a.out = s->hk; a.out_stride = W;
+            break;
+        case dh_batch::D_LBRA:
+            a.M = s->Q1; a.R = W; a.tri = TRI_UPPER; a.epi = EPI_PLAIN;
+            a.x = s->phi; a.x_stride = R; a.out = s->out; a.out_stride = 3 * W;
+            break;
+        default:  // D_LINVT
+            a.M = s->Q2; a.R = R; a.tri = TRI_LOWER; a.epi = EPI_PLAIN;
+            a.x = s->phi; a.x_stride = R; a.out = s->out; a.out_stride = 3 * W; a.out_off = W;
+            break;
+    }
+    return a;
+}
+
+extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int* max_terms) {
+    CHECK_ARG(b, "dh_batch_propagate: null batch");
+    DH_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const double dt = (t_max - t_init) / H_BAR;
+    const int n = b->n, R = b->R, W = b->W;
+    for (auto* s : b->sys) {
+        if (!s->ops_ready || !s->state_set) {
+            dh::set_error("dh_batch_propagate: every system needs operators and wavepackets");
+            return DH_ERR_STATE;
+        }
+        if (s->precision != 0) {
+            dh::set_error("dh_batch_propagate: batches run the FP64 term kernel");
+            return DH_ERR_STATE;
+        }
+    }
+    if (!(dt > 0.0)) return DH_OK;
+    for (auto* s : b->sys) {
+        if (s->phi_dirty) {
+            convert_ao_to_phi(s);
+            s->phi_dirty = false;
+        }
+        int rc = ensure_coefficients(s, dt);
+        if (rc) return rc;
+    }
+    // task table by term count (descending)
+    std::vector<int> Ks(n);
+    for (int i = 0; i < n; ++i) Ks[i] = b->sys[i]->K;
+    if (Ks != b->Ks) {
+        b->Ks = Ks;
+        b->order.resize(n);
+        for (int i = 0; i < n; ++i) b->order[i] = i;
+        std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return Ks[x] > Ks[y]; });
+        std::vector<int> td, tb;
+        b->prefix.assign(1, 0);
+        const int cpb = grouped_cols_per_block(R);
+        for (int q : b->order) {
+            const int nb = (b->sys[q]->N + cpb - 1) / cpb;
+            for (int blk = 0; blk < nb; ++blk) {
+                td.push_back(q);
+                tb.push_back(blk);
+            }
+            b->prefix.push_back((int)td.size());
+        }
+        DH_HIP(hipMemcpyAsync(b->d_task_desc, td.data(), sizeof(int) * td.size(), hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(b->d_task_blk, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
+        b->Kmax = Ks[b->order[0]];
+    }
+    // per-system descriptors (pointers are re-read every step)
+    std::vector<MatvecArgs> args((size_t)dh_batch::NDESC * n);
+    for (int kind = 0; kind < dh_batch::NDESC; ++kind)
+        for (int i = 0; i < n; ++i) args[(size_t)kind * n + i] = batch_desc(b->sys[i], kind);
+    std::vector<NormTask> nt((size_t)2 * n * R);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < R; ++c) {
+            nt[(size_t)i * R + c] = NormTask{b->sys[i]->phi, b->sys[i]->N, R, c, b->d_nval + (size_t)i * R + c};
+            nt[(size_t)n * R + (size_t)i * R + c] =
+                NormTask{b->sys[i]->acc, b->sys[i]->N, R, c, b->d_nval + (size_t)n * R + (size_t)i * R + c};
+        }
+    DH_HIP(hipMemcpyAsync(b->d_args, args.data(), sizeof(MatvecArgs) * args.size(), hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpyAsync(b->d_norms, nt.data(), sizeof(NormTask) * nt.size(), hipMemcpyHostToDevice, st));
+    launch_grouped_norms(b->d_norms, n * R, st);
+    // Chebyshev terms: systems with K_s >= k are the first `active` of `order`
+    int active = n;
+    for (int k = 1; k <= b->Kmax; ++k) {
+        while (active > 0 && Ks[b->order[active - 1]] < k) --active;
+        const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
+        launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k, st);
+    }
+    launch_grouped_norms(b->d_norms + (size_t)n * R, n * R, st);
+    std::vector<double> nv((size_t)2 * n * R);
+    DH_HIP(hipMemcpyAsync(nv.data(), b->d_nval, sizeof(double) * nv.size(), hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    std::vector<int> ok(n, 1);
+    double maxdrift = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < R; ++c) {
+            const double n0 = nv[(size_t)i * R + c], n1 = nv[(size_t)n * R + (size_t)i * R + c];
+            const double d = n0 > 0 ? std::fabs(n1 - n0) / n0 : 0.0;
+            if (!(d < 1e-9)) ok[i] = 0;
+            if (std::isfinite(d)) maxdrift = std::max(maxdrift, d);
+        }
+    // commit acc -> phi for systems whose norm held
+    std::vector<CopyTask> ct;
+    for (int i = 0; i < n; ++i)
+        if (ok[i]) ct.push_back(CopyTask{b->sys[i]->acc, b->sys[i]->phi, (long)b->sys[i]->N * R});
+    DH_HIP(hipMemcpyAsync(b->d_copy, ct.data(), sizeof(CopyTask) * ct.size(), hipMemcpyHostToDevice, st));
+    launch_grouped_copy(b->d_copy, (int)ct.size(), st);
+    // outputs of every system
+    const int all = b->prefix[n];
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_ENERGY * n, b->d_task_desc, b->d_task_blk, all, W, 0, st);
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LBRA * n, b->d_task_desc, b->d_task_blk, all, W, 0, st);
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LINVT * n, b->d_task_desc, b->d_task_blk, all, R, 0, st);
+    std::vector<ObsTask> ot((size_t)n * W);
+    for (int i = 0; i < n; ++i) {
+        dh_system* s = b->sys[i];
+        for (int w = 0; w < W; ++w)
+            ot[(size_t)i * W + w] = ObsTask{s->N, W, s->n_frag, w, s->frag, s->out, s->phi, s->hk, s->pops, s->erg, s->norms};
+    }
+    DH_HIP(hipMemcpyAsync(b->d_obs, ot.data(), sizeof(ObsTask) * ot.size(), hipMemcpyHostToDevice, st));
+    launch_grouped_observables(b->d_obs, n * W, st);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+        dh_system* s = b->sys[i];
+        if (ok[i]) {
+            s->stats[0] = s->K + 1;
+            s->stats[1] += 1;
+            s->stats[2] += s->K + 3;
+            continue;
+        }
+        // bounds did not enclose this system's spectrum: redo it alone with widening
+        s->widen *= 1.25;
+        s->stats[3] += 1;
+        b->stats[3] += 1;
+        int rc = dh_propagate(s, t_init, t_max, nullptr);
+        if (rc) return rc;
+    }
+    b->stats[0] += 1;
+    b->stats[1] += b->Kmax;
+    b->stats[2] = maxdrift;
+    if (max_terms) *max_terms = b->Kmax + 1;
     return DH_OK;
 }

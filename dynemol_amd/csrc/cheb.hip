@@ -19,6 +19,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void matvec_kernel(MatvecArgs a) {
 // PIPE = 0: let the compiler schedule the tile; 1: read x one row ahead and
 // fence the scheduler per row (bounds register use); 2: fence per row, no lookahead.
 template <int R, int CPW, int NW, int JC, int PIPE = 0>
-__global__ __launch_bounds__(NW * 64) void matvec_cols_kernel(MatvecArgs a) {
+__device__ __forceinline__ void cols_body(const MatvecArgs& a, const int block) {
     static_assert(JC % 64 == 0 && JC >= 64, "tile height must be a multiple of 64");
     static_assert(CPW * R <= 64, "one lane per output in the epilogue");
     constexpr int NT = NW * 64;
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(NW * 64) void matvec_cols_kernel(MatvecArgs a) {
     __shared__ double xs[2][2 * R][JC];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int NC = NW * CPW;
-    const int c0 = blockIdx.x * NC;
+    const int c0 = block * NC;
     const int i_base = c0 + wave * CPW;
     int jb = 0, je = a.N;
     if (a.tri == TRI_LOWER) jb = (c0 / JC) * JC;
@@ -337,6 +338,22 @@ if constexpr (PIPE == 0) {
     a.out[oi] = T;
 }
 
+template <int R, int CPW, int NW, int JC, int PIPE = 0>
+__global__ __launch_bounds__(NW * 64) void matvec_cols_kernel(MatvecArgs a) {
+    cols_body<R, CPW, NW, JC, PIPE>(a, blockIdx.x);
+}
+
+// Grouped form: one launch covers many systems; workgroup -> (descriptor, column block)
+// through a task table, and every descriptor's k is overridden by the launch's k.
+template <int R, int CPW, int NW, int JC, int PIPE = 0>
+__global__ __launch_bounds__(NW * 64) void matvec_cols_grouped_kernel(const MatvecArgs* __restrict__ args,
+                                                                      const int* __restrict__ task_desc,
+                                                                      const int* __restrict__ task_blk, int k) {
+    MatvecArgs a = args[task_desc[blockIdx.x]];
+    a.k = k;
+    cols_body<R, CPW, NW, JC, PIPE>(a, task_blk[blockIdx.x]);
+}
+
 // Two-kernel split-K: phase A writes one partial sum per (row split, column)
 // with plain stores; phase B (next launch, so the kernel boundary orders the
 // stores) adds the partials in split order and applies the epilogue.
@@ -444,6 +461,264 @@ __global__ void matvec_reduce_epilogue_kernel(MatvecArgs a) {
     a.out[oi] = T;
 }
 
+// Multi-RHS term as a GEMM on matrix cores: Y^T (C x N) = X^T (C x N) * M
+// with C = 2R real columns (the interleaved complex state read as doubles).
+// 64 x 64 output tiles (real columns x orbitals), 4 waves in 2 x 2, each wave
+// 2 x 2 MFMA 16 x 16 blocks; 16-deep LDS stages; the Chebyshev epilogue runs
+// from an LDS copy of the tile so that re/im of one element meet in a thread.
+// PREC 0: FP64 MFMA 16x16x4 (M in FP64); PREC 1: FP32 MFMA 16x16x4 with FP32
+// accumulation over the whole k range (M32 in FP32, X rounded to FP32 when
+// staged); the recurrence, coefficients and accumulators stay FP64.
+template <int PREC>
+__global__ __launch_bounds__(256) void cheb_gemm_kernel(MatvecArgs a) {
+    constexpr int BK = 16, PD = 65;
+    __shared__ double As[BK][PD];  // X^T tile: [k = j][m = real column]
+    __shared__ double Bs[BK][PD];  // M tile:   [k = j][n = orbital i]
+    __shared__ double Ys[64][PD];  // result tile [real column][orbital]
+    const int C = 2 * a.R;
+    const int c0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const double* X = reinterpret_cast<const double*>(a.x);
+    const long xs = 2L * a.x_stride;  // doubles per state row
+    const long xo = 2L * a.x_off;
+    v4d acc64[2][2];
+    v4f acc32[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            acc64[p][q] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc32[p][q] = (v4f){0.f, 0.f, 0.f, 0.f};
+        }
+    for (int k0 = 0; k0 < a.N; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int mm = t & 63, kk = (t >> 6) + 4 * r;
+            const int gj = k0 + kk, gc = c0 + mm;
+            double v = 0.0;
+            if (gj < a.N && gc < C) v = X[gj * xs + xo + gc];
+            As[kk][mm] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kk = t & 15, nn = (t >> 4) + 16 * r;
+            const int gj = k0 + kk, gi = i0 + nn;
+            double v = 0.0;
+            if (gj < a.N && gi < a.N) {
+                if (PREC == 0) v = a.M[gj + (long)a.ld * gi];
+                else v = (double)a.M32[gj + (long)a.ld * gi];
+            }
+            Bs[kk][nn] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double av[2], bv[2];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) av[p] = As[ks + (lane >> 4)][wm + p * 16 + (lane & 15)];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) bv[q] = Bs[ks + (lane >> 4)][wn + q * 16 + (lane & 15)];
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    if (PREC == 0)
+                        acc64[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[p], bv[q], acc64[p][q], 0, 0, 0);
+                    else
+                        acc32[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[p], (float)bv[q], acc32[p][q],
+                                                                            0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // accumulator -> LDS tile.  f64 layout: value v of lane l is D[l/16 + 4v][l%16];
+    // f32 layout: D[4*(l/16) + v][l%16].
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = PREC == 0 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+                Ys[wm + p * 16 + row][wn + q * 16 + (lane & 15)] =
+                    PREC == 0 ? acc64[p][q][v] : (double)acc32[p][q][v];
+            }
+    __syncthreads();
+    // epilogue over (orbital, complex column) pairs of the tile
+    const int rc = min(32, a.R - c0 / 2);  // complex columns in this tile
+    for (int e = t; e < 64 * rc; e += 256) {
+        const int ii = e / rc, rr = e % rc;
+        const int i = i0 + ii, r = c0 / 2 + rr;
+        if (i >= a.N) continue;
+        const cplx y{Ys[2 * rr][ii], Ys[2 * rr + 1][ii]};
+        const long oi = (long)i * a.out_stride + a.out_off + r;
+        if (a.epi == EPI_PLAIN) {
+            a.out[oi] = y;
+            continue;
+        }
+        const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+        cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+        if (a.epi == EPI_CHEB) {
+            const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+            T.re -= x2.re;
+            T.im -= x2.im;
+            const cplx add = cmul(a.coef[a.k], T);
+            cplx ac = a.acc[oi];
+            ac.re += add.re;
+            ac.im += add.im;
+            a.acc[oi] = ac;
+        } else {
+            const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+            a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+        }
+        a.out[oi] = T;
+    }
+}
+
+// Pipelined multi-RHS term GEMM: 64 real columns x BN orbitals per workgroup,
+// 32-deep k stages double-buffered in LDS with the next stage prefetched into
+// registers while the current one feeds the matrix cores.  BN = 32: waves in
+// 2 (columns) x 2 (orbitals), two 16x16 accumulators each; BN = 16: four waves
+// of 16 columns.  LDS tiles hold FP64 (PREC 0) or FP32 (PREC 1) operands.
+template <int PREC, int BN>
+__global__ __launch_bounds__(256) void cheb_gemm2_kernel(MatvecArgs a) {
+    using T = typename std::conditional<PREC == 0, double, float>::type;
+    constexpr int BM = 64, BK = 32;
+    constexpr int XPT = BK * BM / 256;  // X elements per thread per stage
+    constexpr int HPT = BK * BN / 256;  // H elements per thread per stage
+    constexpr int NACC = BN == 32 ? 2 : 1;
+    __shared__ T As[2][BK][BM + 1];
+    __shared__ T Bs[2][BK][BN + 1];
+    __shared__ double Ys[BM][BN + 1];
+    const int C = 2 * a.R;
+    const int c0 = blockIdx.x * BM, i0 = blockIdx.y * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wc = BN == 32 ? (w >> 1) * 32 : w * 16;
+    const int wi = BN == 32 ? (w & 1) * 16 : 0;
+    const double* X = reinterpret_cast<const double*>(a.x);
+    const long xs = 2L * a.x_stride, xo = 2L * a.x_off;
+    T xr[XPT], hr[HPT];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = t + 256 * q, kk = e / BM, cc = e % BM;
+            const int gj = k0 + kk, gc = c0 + cc;
+            double v = 0.0;
+            if (gj < a.N && gc < C) v = X[gj * xs + xo + gc];
+            xr[q] = (T)v;
+        }
+#pragma unroll
+        for (int q = 0; q < HPT; ++q) {
+            const int e = t + 256 * q, ii = e / BK, kk = e % BK;
+            const int gj = k0 + kk, gi = i0 + ii;
+            T v = (T)0;
+            if (gj < a.N && gi < a.N) {
+                if constexpr (PREC == 0) v = a.M[gj + (long)a.ld * gi];
+                else v = a.M32[gj + (long)a.ld * gi];
+            }
+            hr[q] = v;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = t + 256 * q;
+            As[buf][e / BM][e % BM] = xr[q];
+        }
+#pragma unroll
+        for (int q = 0; q < HPT; ++q) {
+            const int e = t + 256 * q;
+            Bs[buf][e % BK][e / BK] = hr[q];
+        }
+    };
+    v4d acc64[NACC];
+    v4f acc32[NACC];
+#pragma unroll
+    for (int p = 0; p < NACC; ++p) {
+        acc64[p] = (v4d){0.0, 0.0, 0.0, 0.0};
+        acc32[p] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < a.N; k0 += BK) {
+        const bool more = k0 + BK < a.N;
+        if (more) fetch(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const T bv = Bs[buf][ks + (lane >> 4)][wi + (lane & 15)];
+#pragma unroll
+            for (int p = 0; p < NACC; ++p) {
+                const T av = As[buf][ks + (lane >> 4)][wc + p * 16 + (lane & 15)];
+                if constexpr (PREC == 0)
+                    acc64[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc64[p], 0, 0, 0);
+                else
+                    acc32[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc32[p], 0, 0, 0);
+            }
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int p = 0; p < NACC; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = PREC == 0 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+            Ys[wc + p * 16 + row][wi + (lane & 15)] = PREC == 0 ? acc64[p][v] : (double)acc32[p][v];
+        }
+    __syncthreads();
+    const int rc = min(BM / 2, a.R - c0 / 2);
+    for (int e = t; e < BN * rc; e += 256) {
+        const int ii = e / rc, rr = e % rc;
+        const int i = i0 + ii, r = c0 / 2 + rr;
+        if (i >= a.N) continue;
+        const cplx y{Ys[2 * rr][ii], Ys[2 * rr + 1][ii]};
+        const long oi = (long)i * a.out_stride + a.out_off + r;
+        if (a.epi == EPI_PLAIN) {
+            a.out[oi] = y;
+            continue;
+        }
+        const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+        cplx Tk{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+        if (a.epi == EPI_CHEB) {
+            const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+            Tk.re -= x2.re;
+            Tk.im -= x2.im;
+            const cplx add = cmul(a.coef[a.k], Tk);
+            cplx ac = a.acc[oi];
+            ac.re += add.re;
+            ac.im += add.im;
+            a.acc[oi] = ac;
+        } else {
+            const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], Tk);
+            a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+        }
+        a.out[oi] = Tk;
+    }
+}
+
+static int g_gemm_bn = 0;  // 0: choose by size; 16 or 32 forces the tile width
+void set_gemm_tile(int bn) { g_gemm_bn = bn; }
+
+void launch_cheb_gemm(const MatvecArgs& a, int prec, hipStream_t st) {
+    const int C = 2 * a.R;
+    const int ctiles = (C + 63) / 64;
+    int bn = g_gemm_bn;
+    if (bn == 0) bn = ctiles * ((a.N + 31) / 32) >= 256 ? 32 : 16;
+    if (bn == 32) {
+        dim3 grid(ctiles, (a.N + 31) / 32);
+        if (prec == 0) hipLaunchKernelGGL((cheb_gemm2_kernel<0, 32>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((cheb_gemm2_kernel<1, 32>), grid, dim3(256), 0, st, a);
+    } else {
+        dim3 grid(ctiles, (a.N + 15) / 16);
+        if (prec == 0) hipLaunchKernelGGL((cheb_gemm2_kernel<0, 16>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((cheb_gemm2_kernel<1, 16>), grid, dim3(256), 0, st, a);
+    }
+}
+
 int matvec_split_for(int N, int R) {
     const int cpw = R <= 4 ? 4 : (R <= 8 ? 2 : 1);
     const int colblocks = (N + MV_WAVES * cpw - 1) / (MV_WAVES * cpw);
@@ -479,9 +754,24 @@ static void launch_mv_cols(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((matvec_cols_kernel<R, CPW, NW, JC, PIPE>), dim3(blocks), dim3(NW * 64), 0, st, a);
 }
 
+int grouped_cols_per_block(int R) { return R <= 4 ? 16 : (R <= 8 ? 16 : 8); }
+
+void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, const int* d_task_blk, int ntasks, int R,
+                           int k, hipStream_t st) {
+    if (ntasks <= 0) return;
+    switch (R) {
+        case 1: hipLaunchKernelGGL((matvec_cols_grouped_kernel<1, 2, 8, 512, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        case 2: hipLaunchKernelGGL((matvec_cols_grouped_kernel<2, 2, 8, 512, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        case 4: hipLaunchKernelGGL((matvec_cols_grouped_kernel<4, 2, 8, 512, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        case 8: hipLaunchKernelGGL((matvec_cols_grouped_kernel<8, 2, 8, 256, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        case 16: hipLaunchKernelGGL((matvec_cols_grouped_kernel<16, 1, 8, 128, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        default: break;
+    }
+}
+
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
 // counters); 1..4 = column-block kernels of different shapes.
-constexpr int MV_VARIANTS = 15;
+constexpr int MV_VARIANTS = 16;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -533,6 +823,12 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
+    if (variant == 15) {
+        // matrix-core GEMM form; triangular shapes use the column-block kernel
+        if (a.tri == TRI_FULL) launch_cheb_gemm(a, 0, st);
+        else launch_matvec_variant(a, 9, st);
+        return;
+    }
     switch (a.R) {
         case 1: launch_mv_r<1>(a, variant, st); break;
         case 2: launch_mv_r<2>(a, variant, st); break;
@@ -752,12 +1048,10 @@ __global__ void bessel_coef_kernel(double x, double phase, int kmax_alloc, doubl
 //   pop_f = Re sum_{i in f} conj(ket_i) bra_i  (data_output.f:242-263, DUAL_bra = conj(ket), DUAL_ket = bra)
 // and <H> = sum_i conj(phi_bra_i) (H_orth phi_ket)_i (ElHl_Chebyshev.f:329-371).
 // out: [bra (W) | AO_bra (W) | ket (W)] per row; phi: [bra (W) | ket (W)]; hk: N x W.
-__global__ __launch_bounds__(256) void observables_kernel(int N, int W, int n_frag, const int* frag, const cplx* out,
-                                                          const cplx* phi, const cplx* hk, double* pops, double* erg,
-                                                          double* norms) {
+__device__ void observables_body(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi,
+                                 const cplx* hk, double* pops, double* erg, double* norms, const int w) {
     constexpr int MAXF = 16;
     __shared__ double sh[(MAXF + 5) * 4];
-    const int w = blockIdx.x;
     double pf[MAXF + 1];
     for (int f = 0; f <= MAXF; ++f) pf[f] = 0.0;
     double er = 0.0, ei = 0.0, nk = 0.0, nb = 0.0;
@@ -799,6 +1093,17 @@ __global__ __launch_bounds__(256) void observables_kernel(int N, int W, int n_fr
     }
 }
 
+__global__ __launch_bounds__(256) void observables_kernel(int N, int W, int n_frag, const int* frag, const cplx* out,
+                                                          const cplx* phi, const cplx* hk, double* pops, double* erg,
+                                                          double* norms) {
+    observables_body(N, W, n_frag, frag, out, phi, hk, pops, erg, norms, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void grouped_observables_kernel(const ObsTask* tasks) {
+    const ObsTask t = tasks[blockIdx.x];
+    observables_body(t.N, t.W, t.n_frag, t.frag, t.out, t.phi, t.hk, t.pops, t.erg, t.norms, t.w);
+}
+
 // Column norms |phi_c|^2 of an N x R complex block (one workgroup per column).
 __global__ __launch_bounds__(256) void column_norms_kernel(int N, int R, const cplx* x, double* out) {
     __shared__ double sh[4];
@@ -814,6 +1119,25 @@ __global__ __launch_bounds__(256) void column_norms_kernel(int N, int R, const c
     if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+__global__ __launch_bounds__(256) void grouped_norms_kernel(const NormTask* tasks) {
+    __shared__ double sh[4];
+    const NormTask t = tasks[blockIdx.x];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < t.N; i += blockDim.x) {
+        const cplx v = t.x[(long)i * t.R + t.c];
+        s += v.re * v.re + v.im * v.im;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *t.out = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ void grouped_copy_kernel(const CopyTask* tasks) {
+    const CopyTask t = tasks[blockIdx.x];
+    for (long i = threadIdx.x; i < t.n; i += blockDim.x) t.dst[i] = t.src[i];
+}
+
 __global__ void zero_u32_kernel(unsigned* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
@@ -822,7 +1146,38 @@ __global__ void zero_cplx_kernel(cplx* p, long n) {
 }
 
 
+// phi[:, c] *= sqrt(n0[c] / n1[c]) (FP64 norm restoration after a mixed-precision step)
+__global__ void renorm_columns_kernel(cplx* x, int N, int R, const double* n0, const double* n1) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)N * R) return;
+    const int c = (int)(e % R);
+    const double f = n1[c] > 0.0 ? sqrt(n0[c] / n1[c]) : 1.0;
+    x[e].re *= f;
+    x[e].im *= f;
+}
+
+__global__ void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 // ------------------------------------------------------------------ launchers
+void launch_grouped_norms(const NormTask* d_tasks, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(grouped_norms_kernel, dim3(n), dim3(256), 0, st, d_tasks);
+}
+void launch_grouped_copy(const CopyTask* d_tasks, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(grouped_copy_kernel, dim3(n), dim3(256), 0, st, d_tasks);
+}
+void launch_grouped_observables(const ObsTask* d_tasks, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(grouped_observables_kernel, dim3(n), dim3(256), 0, st, d_tasks);
+}
+void launch_renorm_columns(cplx* x, int N, int R, const double* n0, const double* n1, hipStream_t st) {
+    const long n = (long)N * R;
+    hipLaunchKernelGGL(renorm_columns_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, N, R, n0, n1);
+}
+void launch_to_f32(const double* a, float* b, long n, hipStream_t st) {
+    hipLaunchKernelGGL(to_f32_kernel, dim3(4096), dim3(256), 0, st, a, b, n);
+}
 void launch_zero_u32(unsigned* p, int n, hipStream_t st) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(64), dim3(256), 0, st, p, n);
 }

@@ -76,6 +76,7 @@ struct MatvecArgs {
     int k;                  // term index for EPI_CHEB
     int x_stride, x_off;    // column selection: uses x[i*x_stride + x_off + r], r < R
     int out_stride, out_off;
+    const float* M32;       // FP32 copy of M for the mixed-precision GEMM variant (may be null)
 };
 int matvec_split_for(int N, int R);
 void launch_matvec(const MatvecArgs& a, hipStream_t st);
@@ -84,6 +85,12 @@ int matvec_variant_for(int N, int R, int tri);
 void matvec_set_tuned(int N, int R, int tri, int variant);
 void matvec_force_variant(int v);
 int matvec_variant_count();
+void launch_cheb_gemm(const MatvecArgs& a, int prec, hipStream_t st);
+// Grouped column-block matvec over many systems (R in {1, 2, 4, 8, 16}); each
+// task is (descriptor index, column block of grouped_cols_per_block(R) columns).
+int grouped_cols_per_block(int R);
+void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, const int* d_task_blk, int ntasks, int R,
+                           int k, hipStream_t st);
 
 void launch_zero_u32(unsigned* p, int n, hipStream_t st);
 void launch_zero_cplx(cplx* p, long n, hipStream_t st);
@@ -97,6 +104,32 @@ void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J
 void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi, const cplx* hk,
                         double* pops, double* erg, double* norms, hipStream_t st);
 void launch_column_norms(int N, int R, const cplx* x, double* out, hipStream_t st);
+void launch_renorm_columns(cplx* x, int N, int R, const double* n0, const double* n1, hipStream_t st);
+void launch_to_f32(const double* a, float* b, long n, hipStream_t st);
 void launch_identity_pad(double* S, double* h, int N, int ld, hipStream_t st);
+
+
+// Small grouped helpers for multi-system batches.
+struct NormTask {
+    const cplx* x;
+    int N, R, c;
+    double* out;
+};
+struct CopyTask {
+    const cplx* src;
+    cplx* dst;
+    long n;
+};
+struct ObsTask {
+    int N, W, n_frag, w;
+    const int* frag;
+    const cplx* out;
+    const cplx* phi;
+    const cplx* hk;
+    double *pops, *erg, *norms;
+};
+void launch_grouped_norms(const NormTask* d_tasks, int n, hipStream_t st);
+void launch_grouped_copy(const CopyTask* d_tasks, int n, hipStream_t st);
+void launch_grouped_observables(const ObsTask* d_tasks, int n, hipStream_t st);
 
 }  // namespace dh

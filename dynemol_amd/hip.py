@@ -31,7 +31,8 @@ SYMBOLS = [
     "dh_last_error", "dh_version", "dh_init", "dh_finalize", "dh_system_create", "dh_system_destroy",
     "dh_system_bytes", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
-    "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_propagation_elhl",
+    "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
+    "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing"]
@@ -43,7 +44,8 @@ _LIVE = weakref.WeakSet()
 
 @atexit.register
 def _release_all():
-    for obj in sorted(list(_LIVE), key=lambda o: 0 if isinstance(o, System) else 1):
+    rank = {"Batch": 0, "System": 1}
+    for obj in sorted(list(_LIVE), key=lambda o: rank.get(type(o).__name__, 2)):
         try:
             obj.close()
         except Exception:
@@ -89,6 +91,10 @@ def lib():
     L.dh_test_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
+    L.dh_set_precision.argtypes = [_vp, ctypes.c_int]
+    L.dh_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
+    L.dh_batch_destroy.argtypes = [_vp]
+    L.dh_batch_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
     L.dh_test_force_matvec_variant.argtypes = [ctypes.c_int]
     L.dh_test_matvec_timing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
     _lib = L
@@ -257,6 +263,9 @@ class System:
         keys = ["term", "energy", "L_phi_bra", "LinvT_phi", "LT_ket", "Linv_bra", "lanczos"]
         return dict(zip(keys, out.tolist()))
 
+    def set_precision(self, mixed: bool):
+        _check(lib().dh_set_precision(self.h, 1 if mixed else 0), "dh_set_precision")
+
     def set_timing(self, on=True):
         _check(lib().dh_set_timing(self.h, 1 if on else 0), "dh_set_timing")
 
@@ -281,6 +290,28 @@ def run_cholesky(S):
     _check(lib().dh_test_cholesky(n, A.ctypes.data_as(_d), X.ctypes.data_as(_d), ctypes.byref(info)),
            "dh_test_cholesky")
     return A, X, info.value
+
+
+class Batch:
+    """Many systems advanced together (one grouped launch per Chebyshev term)."""
+
+    def __init__(self, ctx: Context, systems):
+        self.ctx = ctx
+        self.systems = list(systems)
+        arr = (_vp * len(self.systems))(*[s.h for s in self.systems])
+        self.h = _vp()
+        _check(lib().dh_batch_create(ctx.h, len(self.systems), arr, ctypes.byref(self.h)), "dh_batch_create")
+        _LIVE.add(self)
+
+    def propagate(self, t_init, t_max):
+        n = ctypes.c_int(0)
+        _check(lib().dh_batch_propagate(self.h, t_init, t_max, ctypes.byref(n)), "dh_batch_propagate")
+        return n.value
+
+    def close(self):
+        if self.h:
+            lib().dh_batch_destroy(self.h)
+            self.h = _vp()
 
 
 def force_matvec_variant(v):

@@ -84,6 +84,12 @@ int dh_get_bounds(dh_system *sys, double *out6);
 /* Override the Lanczos step count and relative safety margin (defaults 48, 0.02). */
 int dh_set_lanczos(dh_system *sys, int steps, double rel_margin);
 
+/* Arithmetic of the Chebyshev term products: 0 = FP64 (default); 1 = mixed:
+ * FP32 matrix-core products with FP32 accumulation on an FP32 copy of H_orth,
+ * while the recurrence, Bessel coefficients, accumulators, norms and
+ * observables stay FP64.  No counterpart in the reference (all FP64). */
+int dh_set_precision(dh_system *sys, int mode);
+
 /* Wavepacket wp's AO-basis Psi_t_bra and Psi_t_ket, N complex each. */
 int dh_set_wavepacket(dh_system *sys, int wp, const double *psi_bra, const double *psi_ket);
 int dh_get_wavepacket(dh_system *sys, int wp, double *psi_bra, double *psi_ket);
@@ -113,6 +119,18 @@ int dh_set_timing(dh_system *sys, int on);
  * L^-1 bra, Lanczos (1 RHS)].  No counterpart in the reference (its GEMV tile
  * sizes are compile-time macros, dzgemv_kernels.cu:581-608). */
 int dh_matvec_tuning(dh_system *sys, int *out7);
+
+/*
+ * Multi-system batch: n systems of possibly different N with the same number
+ * of wavepackets (1, 2, 4 or 8), advanced together by one grouped launch per
+ * Chebyshev term.  The systems stay owned by the caller; results are read per
+ * system with dh_get_wavepacket / dh_get_observables.  No counterpart in the
+ * reference (one system per process, ElHl_Chebyshev.f:148-291).
+ */
+typedef struct dh_batch dh_batch;
+int dh_batch_create(dh_context *ctx, int n, dh_system *const *systems, dh_batch **batch);
+int dh_batch_destroy(dh_batch *batch);
+int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_terms);
 
 /*
  * Drop-in for propagationelhl_gpucaller_ (Taylor_gpu.cpp:219-232, 634-736):

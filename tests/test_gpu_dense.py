@@ -82,7 +82,7 @@ def test_cholesky_reports_non_spd():
     assert info == 71
 
 
-@pytest.mark.parametrize("variant", list(range(15)))
+@pytest.mark.parametrize("variant", list(range(16)))
 @pytest.mark.parametrize("n_wp", [1, 2, 3])
 def test_matvec_variants_propagate_exactly(variant, n_wp):
     """Every matvec kernel variant the autotuner may pick gives the exact propagator."""
