@@ -1086,7 +1086,8 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
         CHECK_ARG(systems[i]->W == W, "dh_batch_create: all systems need the same wavepacket count");
     }
     const int R = 2 * W;
-    CHECK_ARG(R == 2 || R == 4 || R == 8 || R == 16, "dh_batch_create: wavepackets per system must be 1, 2, 4 or 8");
+    CHECK_ARG(R == 2 || R == 4 || R == 8 || R == 16 || R == 32,
+              "dh_batch_create: wavepackets per system must be 1, 2, 4, 8 or 16");
     DH_HIP(hipSetDevice(ctx->device));
     dh_batch* b = new dh_batch();
     b->ctx = ctx;
@@ -1095,7 +1096,8 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
     b->R = R;
     b->n = n;
     long tasks = 0;
-    for (auto* s : b->sys) tasks += (s->N + grouped_cols_per_block(R) - 1) / grouped_cols_per_block(R);
+    const int cpb0 = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
+    for (auto* s : b->sys) tasks += (s->N + cpb0 - 1) / cpb0;
     int rc;
 #define BALLOC(p, c)                                    \
     do {                                                \
@@ -1196,7 +1198,7 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
         std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return Ks[x] > Ks[y]; });
         std::vector<int> td, tb;
         b->prefix.assign(1, 0);
-        const int cpb = grouped_cols_per_block(R);
+        const int cpb = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
         for (int q : b->order) {
             const int nb = (b->sys[q]->N + cpb - 1) / cpb;
             for (int blk = 0; blk < nb; ++blk) {

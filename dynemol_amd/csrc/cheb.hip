@@ -754,7 +754,7 @@ static void launch_mv_cols(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((matvec_cols_kernel<R, CPW, NW, JC, PIPE>), dim3(blocks), dim3(NW * 64), 0, st, a);
 }
 
-int grouped_cols_per_block(int R) { return R <= 4 ? 16 : (R <= 8 ? 16 : 8); }
+int grouped_cols_per_block(int R) { return R <= 8 ? 16 : 8; }
 
 void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, const int* d_task_blk, int ntasks, int R,
                            int k, hipStream_t st) {
@@ -765,6 +765,7 @@ void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, con
         case 4: hipLaunchKernelGGL((matvec_cols_grouped_kernel<4, 2, 8, 512, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
         case 8: hipLaunchKernelGGL((matvec_cols_grouped_kernel<8, 2, 8, 256, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
         case 16: hipLaunchKernelGGL((matvec_cols_grouped_kernel<16, 1, 8, 128, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
+        case 32: hipLaunchKernelGGL((matvec_cols_grouped_kernel<32, 1, 8, 64, 1>), dim3(ntasks), dim3(512), 0, st, d_args, d_task_desc, d_task_blk, k); break;
         default: break;
     }
 }

@@ -122,7 +122,7 @@ int dh_matvec_tuning(dh_system *sys, int *out7);
 
 /*
  * Multi-system batch: n systems of possibly different N with the same number
- * of wavepackets (1, 2, 4 or 8), advanced together by one grouped launch per
+ * of wavepackets (1, 2, 4, 8 or 16), advanced together by one grouped launch per
  * Chebyshev term.  The systems stay owned by the caller; results are read per
  * system with dh_get_wavepacket / dh_get_observables.  No counterpart in the
  * reference (one system per process, ElHl_Chebyshev.f:148-291).
