@@ -1287,3 +1287,33 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     if (max_terms) *max_terms = b->Kmax + 1;
     return DH_OK;
 }
+
+// preprocess_ElHl_Chebyshev (ElHl_Chebyshev.f:110-129): Psi_ket = C_ket, Psi_bra = S C_bra,
+// i.e. phi_ket = L^T C_ket and phi_bra = L^-1 S C_bra = L^T C_bra, then the t = 0 outputs.
+extern "C" int dh_set_initial_state(dh_system* s, int wp, const double* ao_bra, const double* ao_ket) {
+    CHECK_ARG(s && ao_bra && ao_ket && wp >= 0 && wp < s->W, "dh_set_initial_state: bad argument");
+    CHECK_ARG(s->ops_ready, "dh_set_initial_state: operators not built");
+    DH_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const int W = s->W, R = 2 * W;
+    if (s->phi_dirty && s->state_set) {  // bring the other wavepackets into the current basis first
+        convert_ao_to_phi(s);
+    }
+    s->phi_dirty = false;
+    // stage C_bra, C_ket in the hk scratch (N x W is enough for one column pair when W >= 2; use x1 otherwise)
+    cplx* stage = s->x1;  // N x R scratch
+    DH_HIP(hipMemcpy2DAsync(stage + 0, sizeof(cplx) * R, ao_bra, sizeof(cplx), sizeof(cplx), s->N, hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpy2DAsync(stage + 1, sizeof(cplx) * R, ao_ket, sizeof(cplx), sizeof(cplx), s->N, hipMemcpyHostToDevice, st));
+    MatvecArgs a{};
+    a.ld = s->ld; a.N = s->N; a.partial = s->partial; a.counters = s->counters; a.epi = EPI_PLAIN;
+    a.M = s->Q1; a.tri = TRI_LOWER; a.R = 1; a.split = matvec_split_for(s->N, 1);
+    a.x = stage; a.x_stride = R; a.x_off = 0; a.out = s->phi; a.out_stride = R; a.out_off = wp;
+    launch_matvec(a, st);
+    a.x_off = 1; a.out_off = W + wp;
+    launch_matvec(a, st);
+    launch_outputs(s);
+    DH_HIP(hipGetLastError());
+    DH_HIP(hipStreamSynchronize(st));
+    s->state_set = true;
+    return DH_OK;
+}

@@ -1,0 +1,69 @@
+! Replays the reference's slice_Cheb step sequence (Chebyshev_driver.f:102-169:
+! preprocess, then one ElHl_Chebyshev call per nuclear step) through the Fortran
+! drop-in module.  Input deck (text): see dynemol_amd/fortran/deck.py.
+program dh_harness
+    use dh_types_m
+    use gpu_ElHl_Chebyshev_m
+    implicit none
+    type(structure) :: system
+    type(STO_basis), allocatable :: basis(:)
+    type(f_time) :: QDyn
+    complex*16, allocatable :: AO_bra(:, :), AO_ket(:, :), Dual_bra(:, :), Dual_ket(:, :)
+    integer :: N, na, nf, nsteps, i, a, it, p, rebuild_every
+    real*8 :: delta_t, t, re, im, jitter(3)
+    character(len=512) :: fin, fout
+
+    call get_command_argument(1, fin)
+    call get_command_argument(2, fout)
+    open(10, file=trim(fin), status='old', action='read')
+    read(10, *) na, N, nf, nsteps, delta_t, rebuild_every
+    system%atoms = na
+    allocate(system%coord(na, 3), system%fragment(na), basis(N))
+    do a = 1, na
+        read(10, *) system%coord(a, 1), system%coord(a, 2), system%coord(a, 3), system%fragment(a)
+    end do
+    do i = 1, N
+        read(10, *) basis(i)%atom, basis(i)%n, basis(i)%l, basis(i)%m, basis(i)%Nzeta, basis(i)%zeta(1), &
+                    basis(i)%zeta(2), basis(i)%coef(1), basis(i)%coef(2), basis(i)%IP, basis(i)%k_WH, basis(i)%V_shift
+        basis(i)%fragment = system%fragment(basis(i)%atom)
+    end do
+    allocate(AO_bra(N, 2), AO_ket(N, 2), Dual_bra(N, 2), Dual_ket(N, 2))
+    do p = 1, 2
+        do i = 1, N
+            read(10, *) re, im
+            AO_ket(i, p) = cmplx(re, im, kind=8)
+        end do
+    end do
+    AO_bra = AO_ket
+    QDyn%n_fragments = nf
+    allocate(QDyn%dyn(nsteps + 1, 0:nf + 1, 2))
+    QDyn%dyn = 0.d0
+
+    call gpu_preprocess_ElHl_Chebyshev(system, basis, AO_bra, AO_ket, Dual_bra, Dual_ket, QDyn, 1)
+    t = 0.d0
+    do it = 2, nsteps + 1
+        if (rebuild_every > 0) then
+            if (mod(it - 1, rebuild_every) == 0) then   ! nuclei move: displacement read from the deck
+                do a = 1, na
+                    read(10, *) jitter
+                    system%coord(a, :) = system%coord(a, :) + jitter
+                end do
+            end if
+        end if
+        call gpu_ElHl_Chebyshev(system, basis, AO_bra, AO_ket, Dual_bra, Dual_ket, QDyn, t, delta_t, it)
+    end do
+    close(10)
+
+    open(11, file=trim(fout), status='replace', action='write')
+    do it = 1, nsteps + 1
+        write(11, '(*(es24.16))') ((QDyn%dyn(it, i, p), i = 0, nf + 1), p = 1, 2)
+    end do
+    do p = 1, 2
+        do i = 1, N       ! final Psi_ket (= AO_ket) and Psi_bra (= Dual_ket)
+            write(11, '(4es24.16)') real(AO_ket(i, p)), aimag(AO_ket(i, p)), real(Dual_ket(i, p)), aimag(Dual_ket(i, p))
+        end do
+    end do
+    write(11, '(4es24.16)') real(QM_wp_erg(1)), aimag(QM_wp_erg(1)), real(QM_wp_erg(2)), aimag(QM_wp_erg(2))
+    close(11)
+    call gpu_ElHl_finalize()
+end program dh_harness

@@ -32,7 +32,7 @@ SYMBOLS = [
     "dh_system_bytes", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
-    "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate",
+    "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing"]
@@ -92,6 +92,7 @@ def lib():
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
     L.dh_set_precision.argtypes = [_vp, ctypes.c_int]
+    L.dh_set_initial_state.argtypes = [_vp, ctypes.c_int, _d, _d]
     L.dh_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.dh_batch_destroy.argtypes = [_vp]
     L.dh_batch_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
@@ -231,6 +232,12 @@ class System:
         k = np.ascontiguousarray(psi_ket, dtype=np.complex128)
         assert b.shape == (self.N,) and k.shape == (self.N,)
         _check(lib().dh_set_wavepacket(self.h, wp, b.ctypes.data_as(_d), k.ctypes.data_as(_d)), "dh_set_wavepacket")
+
+    def set_initial_state(self, wp, ao_bra, ao_ket):
+        b = np.ascontiguousarray(ao_bra, dtype=np.complex128)
+        k = np.ascontiguousarray(ao_ket, dtype=np.complex128)
+        _check(lib().dh_set_initial_state(self.h, wp, b.ctypes.data_as(_d), k.ctypes.data_as(_d)),
+               "dh_set_initial_state")
 
     def wavepacket(self, wp):
         b = np.zeros(self.N, dtype=np.complex128)

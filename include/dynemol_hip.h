@@ -93,6 +93,11 @@ int dh_set_precision(dh_system *sys, int mode);
 /* Wavepacket wp's AO-basis Psi_t_bra and Psi_t_ket, N complex each. */
 int dh_set_wavepacket(dh_system *sys, int wp, const double *psi_bra, const double *psi_ket);
 int dh_get_wavepacket(dh_system *sys, int wp, double *psi_bra, double *psi_ket);
+/* Initial state from AO coefficients (the FMO/eigenvector C the caller
+ * chose): Psi_ket = C_ket, Psi_bra = S C_bra, as preprocess_ElHl_Chebyshev
+ * forms them (ElHl_Chebyshev.f:110-129); also computes the t = 0 observables.
+ * Needs built operators. */
+int dh_set_initial_state(dh_system *sys, int wp, const double *ao_bra, const double *ao_ket);
 /*
  * Propagate every wavepacket from t_init to t_max (ps): Psi_ket <- exp(-i
  * S^-1 h dt / hbar) Psi_ket and Psi_bra <- exp(-i h S^-1 dt / hbar) Psi_bra,

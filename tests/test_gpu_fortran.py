@@ -1,0 +1,53 @@
+"""The Fortran drop-in (ISO_C_BINDING module + gpu_ElHl_Chebyshev_m) replays the step sequence == oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from dynemol_amd.fortran.deck import read_output, write_deck
+from dynemol_amd.structures import ch_cluster
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "dynemol_amd", "fortran", "dh_harness")
+DT = 5e-4
+
+
+@pytest.mark.parametrize("rebuild_every", [0, 5])
+def test_fortran_harness_matches_oracle(tmp_path, rebuild_every):
+    print(f"[gpu] fortran harness rebuild_every={rebuild_every}", flush=True)
+    st = ch_cluster(seed=1)
+    b = st.basis()
+    S = O.overlap(st.coords, b)
+    h = O.huckel(b, S)
+    E, C = sl.eigh(h, S)
+    nsteps = 12
+    rng = np.random.default_rng(9)
+    disp = [rng.normal(0, 0.02, st.coords.shape) for _ in range(nsteps // max(1, rebuild_every))] if rebuild_every else []
+    deck = tmp_path / "deck.txt"
+    out = tmp_path / "out.txt"
+    write_deck(str(deck), st, b, C[:, 208], C[:, 207], nsteps, DT, rebuild_every, disp)
+    r = subprocess.run([HARNESS, str(deck), str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dyn, bra, ket, erg = read_output(str(out), b.N, 2, nsteps)
+
+    ref = O.ElHlReference(b, 2, np.stack([S @ C[:, 208], S @ C[:, 207]], 1), np.stack([C[:, 208], C[:, 207]], 1) + 0j,
+                          err_tol=1e-13, norm_tol=1e-11)
+    coords = st.coords.copy()
+    t = 0.0
+    d = 0
+    for it in range(2, nsteps + 2):
+        if rebuild_every and (it - 1) % rebuild_every == 0:
+            coords = coords + disp[d]
+            d += 1
+        t, o = ref.step(coords, t, DT, it)
+        for p in range(2):
+            assert np.abs(dyn[it - 1, p, 1:] - o["pops"][:, p]).max() < 1e-10, (it, p)
+            assert abs(dyn[it - 1, p, 0] - t) < 1e-15
+    for p in range(2):
+        assert np.abs(ket[p] - ref.psi_ket[:, p]).max() <= 1e-9 * np.abs(ket[p]).max()
+        assert np.abs(bra[p] - ref.psi_bra[:, p]).max() <= 1e-9 * np.abs(bra[p]).max()
+        assert abs(erg[p] - o["erg"][p]) < 1e-8
