@@ -102,6 +102,74 @@ def run_gpu(args, st, b, S, lumo, homo):
     }
 
 
+def run_batched(st, b, S, n_wp, steps, seed=11):
+    """n_wp wavepackets on the N = 4096 system through the multi-RHS term kernel, FP64 and mixed."""
+    from dynemol_amd import hip
+    rng = np.random.default_rng(seed)
+    kets = []
+    for _ in range(n_wp):
+        k = rng.normal(size=b.N) + 1j * rng.normal(size=b.N)
+        kets.append(k / np.sqrt(np.real(np.conj(k) @ (S @ k))))
+    out = {}
+    with hip.Context(0) as ctx:
+        for mixed in (False, True):
+            s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
+            s.set_precision(mixed)
+            s.build(st.coords)
+            for w, k in enumerate(kets):
+                s.set_wavepacket(w, S @ k, k)
+            t = 0.0
+            s.propagate(t, t + DT_PS)  # warm-up step (also autotunes)
+            t += DT_PS
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.propagate(t, t + DT_PS)
+                t += DT_PS
+            el = time.perf_counter() - t0
+            pops = np.array([s.observables(w)[1] for w in range(n_wp)])
+            out[mixed] = {"value": n_wp * steps / el, "pops": pops, "tuning": s.tuning()["term"],
+                          "terms": s.stats()["terms"]}
+            s.close()
+    err = float(np.abs(out[True]["pops"] - out[False]["pops"]).max())
+    return {"n_orbitals": b.N, "wavepackets": n_wp, "steps": steps + 1,
+            "fp64_value": round(out[False]["value"], 3), "mixed_value": round(out[True]["value"], 3),
+            "fp64_term_variant": out[False]["tuning"], "chebyshev_terms": out[False]["terms"],
+            "mixed_pop_err_vs_fp64": err, "unit": "wavepacket-steps/s"}
+
+
+def run_serve(n_req, seed=5, lam=2.0, n_lo=256, n_hi=8192, budget_gib=96):
+    """Poisson arrivals; N log-uniform, 1-16 wavepackets, 10-200 steps each (one context)."""
+    from dynemol_amd import hip
+    from dynemol_amd.serve import Request, Server, latency_stats
+    from dynemol_amd.structures import heavy_cluster, log_uniform_int
+    from oracle import oracle as O  # request setup: S for S-normalised random initial states
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    for i in range(n_req):
+        N = log_uniform_int(rng, n_lo, n_hi)
+        st = heavy_cluster(int(rng.integers(1 << 30)), N)
+        b = st.basis()
+        S = O.overlap(st.coords, b)
+        w = int(rng.integers(1, 17))
+        kets = []
+        for _ in range(w):
+            k = rng.normal(size=N) + 1j * rng.normal(size=N)
+            kets.append(k / np.sqrt(np.real(np.conj(k) @ (S @ k))))
+        t += rng.exponential(1.0 / lam)
+        reqs.append(Request(rid=i, basis=b, coords=st.coords, psi_bra=[S @ k for k in kets], psi_ket=kets,
+                            steps=int(rng.integers(10, 201)), arrival=t, n_frag=2))
+    log(f"serve: {n_req} requests generated, N in [{min(r.basis.N for r in reqs)}, {max(r.basis.N for r in reqs)}]")
+    with hip.Context(0) as ctx:
+        srv = Server(ctx, hbm_budget_bytes=budget_gib * 2**30)
+        t0 = time.perf_counter()
+        done = srv.run(reqs, log=log)
+        el = time.perf_counter() - t0
+    st = latency_stats(done)
+    st.update({"requests": n_req, "arrival_rate_per_s": lam, "throughput_wp_steps_per_s": st["wavepacket_steps"] / el,
+               "wall_s": el})
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+
+
 def run_cpu_baseline(st, b, S, h, lumo, homo, steps):
     """Reference algorithm (Taylor.f Propagation, tolerance 1e-8) restated in C/OpenMP, timed on the host."""
     import scipy.linalg as sl
@@ -135,6 +203,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-wp", type=int, default=32)
+    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -179,9 +251,15 @@ def main():
         "chebyshev_terms_per_step": int(np.median(g["terms"])),
         "rebuild_ms": round(1e3 * g["rebuild_s"] / max(1, g["rebuilds"]), 3),
         "spectral_bounds_eV": [round(x, 4) for x in g["bounds"][:2]],
-        "mixed": None,
+        "batched": None,
         "serve": None,
     }
+    if not args.no_extra:
+        log("batched leg")
+        out["batched"] = run_batched(st, b, S, args.batched_wp, args.batched_steps)
+        if args.serve_requests > 0:
+            log("serve leg")
+            out["serve"] = run_serve(args.serve_requests)
     if not args.no_cpu and args.cpu_steps > 0:
         log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(st, b, S, h, lumo, homo, args.cpu_steps)

@@ -44,7 +44,10 @@ def test_alone_equals_in_mix():
             i, n, w, k, a = specs[probe] if probe < len(specs) else (7, 350, 4, 5, 0.0)
             alone_req = make_request(i, n, w, k, 0.0, 200 + i if probe < len(specs) else 300, mixed=(probe == 7))
             alone = Server(ctx, hbm_budget_bytes=64 * 2**30).run([alone_req])[0]
-            tol = 0.0 if probe != 7 else 0.0
+            # FP64 requests run through the grouped kernels, whose shape is fixed: bitwise.
+            # The mixed request steps alone; its output matvecs use the variant the
+            # autotuner timed fastest at build, so the summation order may differ.
+            tol = 0.0 if probe != 7 else 1e-13
             for q in range(len(alone.result_ket)):
                 assert np.abs(done[i].result_ket[q] - alone.result_ket[q]).max() <= tol
                 assert np.abs(done[i].pops[q] - alone.pops[q]).max() <= tol
