@@ -474,6 +474,7 @@ static int autotune_system(dh_system* s) {
         a.out = sh.out;
         a.x_stride = sh.xs;
         a.out_stride = sh.os;
+        if (matvec_is_tuned(s->N, sh.R, sh.tri)) continue;
         a.split = matvec_split_for(s->N, sh.R);
         int rc = autotune_shape(s, a);
         if (rc) return rc;

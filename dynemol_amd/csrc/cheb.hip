@@ -21,6 +21,7 @@
 
 #include <type_traits>
 #include <utility>
+#include <mutex>
 #include <vector>
 
 namespace dh {
@@ -804,15 +805,26 @@ static int g_variant_override = -1;
 struct TuneKey {
     int N, R, tri;
 };
+// Process-wide: a shape is tuned once, by the first system that has it, so
+// every later system of that shape runs the same kernels (same summation order).
 static std::vector<std::pair<TuneKey, int>> g_tuned;
+static std::mutex g_tuned_mu;
 
 int matvec_variant_for(int N, int R, int tri) {
     if (g_variant_override >= 0) return g_variant_override;
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
     for (auto& kv : g_tuned)
         if (kv.first.N == N && kv.first.R == R && kv.first.tri == tri) return kv.second;
     return 1;
 }
+bool matvec_is_tuned(int N, int R, int tri) {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    for (auto& kv : g_tuned)
+        if (kv.first.N == N && kv.first.R == R && kv.first.tri == tri) return true;
+    return false;
+}
 void matvec_set_tuned(int N, int R, int tri, int variant) {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
     for (auto& kv : g_tuned)
         if (kv.first.N == N && kv.first.R == R && kv.first.tri == tri) {
             kv.second = variant;

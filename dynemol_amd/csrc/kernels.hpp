@@ -83,6 +83,7 @@ void launch_matvec(const MatvecArgs& a, hipStream_t st);
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st);
 int matvec_variant_for(int N, int R, int tri);
 void matvec_set_tuned(int N, int R, int tri, int variant);
+bool matvec_is_tuned(int N, int R, int tri);
 void matvec_force_variant(int v);
 int matvec_variant_count();
 void launch_cheb_gemm(const MatvecArgs& a, int prec, hipStream_t st);
