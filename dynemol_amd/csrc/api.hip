@@ -33,7 +33,7 @@ struct dh_context {
     hipStream_t stream = nullptr;
     // shared workspace for operator builds (one build at a time per context)
     int ws_ld = 0;
-    double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr, *trwork = nullptr;
+    double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr;
     int* d_info = nullptr;
     const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
     dh_system* dropin = nullptr;
@@ -131,8 +131,8 @@ extern "C" int dh_init(int device, dh_context** out) {
 }
 
 static void free_ws(dh_context* c) {
-    hipFree(c->S); hipFree(c->h); hipFree(c->W); hipFree(c->Hp); hipFree(c->Dinv); hipFree(c->trwork);
-    c->S = c->h = c->W = c->Hp = c->Dinv = c->trwork = nullptr;
+    hipFree(c->S); hipFree(c->h); hipFree(c->W); hipFree(c->Hp); hipFree(c->Dinv);
+    c->S = c->h = c->W = c->Hp = c->Dinv = nullptr;
     c->ws_ld = 0;
     c->ws_owner = nullptr;
 }
@@ -147,7 +147,6 @@ static int ensure_ws(dh_context* c, int ld) {
     DALLOC(c->W, n);
     DALLOC(c->Hp, n);
     DALLOC(c->Dinv, (size_t)ld * TILE);
-    DALLOC(c->trwork, (size_t)ld * TILE);
     c->ws_ld = ld;
     return DH_OK;
 }
@@ -504,7 +503,7 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
         dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
         return DH_ERR_NOT_SPD;
     }
-    launch_trtri_lower(s->Q1, s->Q2, c->Dinv, ld, c->trwork, st);
+    launch_trtri_lower(s->Q1, s->Q2, c->Dinv, ld, c->W, st);  // W is free until W = L^-1 h
     {   // W = L^-1 h
         Gemm g{};
         g.M = ld; g.N = ld; g.K = ld;
@@ -922,7 +921,7 @@ extern "C" int dh_test_cholesky(int n, double* A, double* Linv, int* info) {
     DALLOC(dA, nn);
     DALLOC(dX, nn);
     DALLOC(dD, (long)n * TILE);
-    DALLOC(dw, (long)n * TILE);
+    DALLOC(dw, nn / 2 + TILE);
     DALLOC(di, 1);
     DH_HIP(hipMemcpy(dA, A, sizeof(double) * nn, hipMemcpyHostToDevice));
     DH_HIP(hipMemset(di, 0, sizeof(int)));

@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
     if (g.kmode == K_BELOW_ROWTILE) khi = min(g.K, i0 + TILE);
     else if (g.kmode == K_BELOW_COLTILE) khi = min(g.K, j0 + TILE);
     else if (g.kmode == K_FROM_ROWTILE) klo = i0;
+    else if (g.kmode == K_FROM_COLTILE) klo = j0;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wi = (w >> 1) * 32, wj = (w & 1) * 32;
     v4d acc[2][2];
@@ -105,54 +106,194 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
             }
 }
 
+// The same product on 128 x 128 tiles for large M and N: 4 waves in a 2 x 2
+// arrangement, each wave a 64 x 64 quadrant = 4 x 4 MFMA blocks; a 16-deep k
+// stage is double-buffered in LDS and the next stage is fetched into registers
+// while the current one feeds the matrix cores (one barrier per stage).  The
+// MFMA is issued with the operands swapped (D^T = B^T A^T), so a lane's four
+// results lie in four columns and each column's 16 rows are contiguous in C.
+// lower_only: tiles with row-tile < col-tile are skipped at the 64 x 64 level,
+// as in the 64 x 64 kernel.
+constexpr int G2_T = 128, G2_BK = 16, G2_PAD = G2_T + 4;
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void gemm128_f64_kernel(Gemm g) {
+    __shared__ double As[2][G2_BK][G2_PAD];
+    __shared__ double Bs[2][G2_BK][G2_PAD];
+    const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
+    if (g.lower_only && bi < bj) return;
+    const int i0 = bi * G2_T, j0 = bj * G2_T;
+    const double* A = g.A + z * g.strideA;
+    const double* B = g.B + z * g.strideB;
+    double* C = g.C + z * g.strideC;
+    int klo = 0, khi = g.K;
+    if (g.kmode == K_BELOW_ROWTILE) khi = min(g.K, i0 + G2_T);
+    else if (g.kmode == K_BELOW_COLTILE) khi = min(g.K, j0 + G2_T);
+    else if (g.kmode == K_FROM_ROWTILE) klo = i0;
+    else if (g.kmode == K_FROM_COLTILE) klo = j0;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wi = (w >> 1) * 64, wj = (w & 1) * 64;
+    v4d acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    double ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int ii = TA ? (t >> 4) + 16 * r : (t & 127);
+            const int kk = TA ? (t & 15) : (t >> 7) + 2 * r;
+            const int gi = i0 + ii, gk = k0 + kk;
+            double v = 0.0;
+            if (gi < g.M && gk < khi) v = TA ? A[gk + g.lda * (long)gi] : A[gi + g.lda * (long)gk];
+            ra[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int jj = TB ? (t & 127) : (t >> 4) + 16 * r;
+            const int kk = TB ? (t >> 7) + 2 * r : (t & 15);
+            const int gj = j0 + jj, gk = k0 + kk;
+            double v = 0.0;
+            if (gj < g.N && gk < khi) v = TB ? B[gj + g.ldb * (long)gk] : B[gk + g.ldb * (long)gj];
+            rb[r] = v;
+        }
+    };
+    auto swrite = [&](int s) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int ii = TA ? (t >> 4) + 16 * r : (t & 127);
+            const int kk = TA ? (t & 15) : (t >> 7) + 2 * r;
+            As[s][kk][ii] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int jj = TB ? (t & 127) : (t >> 4) + 16 * r;
+            const int kk = TB ? (t >> 7) + 2 * r : (t & 15);
+            Bs[s][kk][jj] = rb[r];
+        }
+    };
+    int s = 0;
+    if (klo < khi) {
+        gload(klo);
+        swrite(0);
+    }
+    __syncthreads();
+    for (int k0 = klo; k0 < khi; k0 += G2_BK) {
+        const bool more = k0 + G2_BK < khi;
+        if (more) gload(k0 + G2_BK);
+#pragma unroll
+        for (int ks = 0; ks < G2_BK; ks += 4) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[s][ks + (lane >> 4)][wi + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[s][ks + (lane >> 4)][wj + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], acc[a][b], 0, 0, 0);
+        }
+        if (more) swrite(s ^ 1);
+        __syncthreads();
+        s ^= 1;
+    }
+    if (g.lower_only && (i0 + wi) / TILE < (j0 + wj) / TILE) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = i0 + wi + a * 16 + (lane & 15);
+                const int gj = j0 + wj + b * 16 + (lane >> 4) + 4 * v;
+                if (gi < g.M && gj < g.N) {
+                    const long idx = gi + g.ldc * (long)gj;
+                    const double old = g.beta == 0.0 ? 0.0 : g.beta * C[idx];
+                    C[idx] = g.alpha * acc[a][b][v] + old;
+                }
+            }
+}
+
 void launch_gemm(const Gemm& g, hipStream_t st) {
     if (g.M <= 0 || g.N <= 0) return;
-    dim3 grid((g.M + TILE - 1) / TILE, (g.N + TILE - 1) / TILE, g.batch > 0 ? g.batch : 1);
+    const unsigned nz = g.batch > 0 ? g.batch : 1;
+    if (g.M >= 2 * G2_T && g.N >= 2 * G2_T) {
+        dim3 grid((g.M + G2_T - 1) / G2_T, (g.N + G2_T - 1) / G2_T, nz);
+        if (!g.transA && !g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<0, 0>), grid, dim3(256), 0, st, g);
+        else if (!g.transA) hipLaunchKernelGGL((gemm128_f64_kernel<0, 1>), grid, dim3(256), 0, st, g);
+        else if (!g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<1, 0>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((gemm128_f64_kernel<1, 1>), grid, dim3(256), 0, st, g);
+        return;
+    }
+    dim3 grid((g.M + TILE - 1) / TILE, (g.N + TILE - 1) / TILE, nz);
     hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, st, g);
 }
 
 // Factor the 64 x 64 diagonal block at (j0, j0) in place (lower) and write its
-// inverse to Dinv[blk].  One wavefront; the block lives in LDS.
-__global__ __launch_bounds__(64) void potrf_diag_kernel(double* A, int ld, int j0, double* Dinv, int* info) {
-    __shared__ double Ls[TILE][PAD];
-    __shared__ double Xs[TILE][PAD];
-    const int t = threadIdx.x;
-    for (int c = 0; c < TILE; ++c) Ls[t][c] = A[(long)(j0 + t) + (long)ld * (j0 + c)];
+// inverse to Dinv[blk].  1024 threads; thread t owns the four elements
+// (r, c) = (t % 64, t / 64 + 16 q) of the block, held column-major in LDS.
+//   factor : unscaled right-looking elimination, S_rc -= S_rj S_cj / S_jj for
+//            c > j, r >= c (step j reads column j, writes later columns: one
+//            barrier per step), then L_rc = S_rc / sqrt(S_cc);
+//   inverse: L X = I by right-looking forward substitution over rows,
+//            B_rc -= L_ri B_ic / L_ii for r > i, then X_rc = B_rc / L_rr.
+__global__ __launch_bounds__(1024) void potrf_diag_kernel(double* A, int ld, int j0, double* Dinv, int* info) {
+    __shared__ double Ls[TILE * TILE];  // Ls[c * 64 + r]
+    __shared__ double Bs[TILE * TILE];
+    __shared__ int bad;
+    const int t = threadIdx.x, r = t & 63, cq = t >> 6;
+    if (t == 0) bad = TILE + 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = cq + 16 * q;
+        Ls[c * TILE + r] = A[(long)(j0 + r) + (long)ld * (j0 + c)];
+        Bs[c * TILE + r] = r == c ? 1.0 : 0.0;
+    }
     __syncthreads();
-    for (int j = 0; j < TILE; ++j) {
-        double d = Ls[j][j];
-        if (!(d > 0.0)) {
-            if (t == 0 && *info == 0) *info = j0 + j + 1;
-            d = 1.0;
-        }
-        const double djj = sqrt(d);
-        __syncthreads();
-        if (t == j) Ls[j][j] = djj;
-        if (t > j) Ls[t][j] = Ls[t][j] / djj;
-        __syncthreads();
-        if (t > j) {
-            const double lij = Ls[t][j];
-            for (int k = j + 1; k <= t; ++k) Ls[t][k] -= lij * Ls[k][j];
+    for (int j = 0; j < TILE - 1; ++j) {
+        const double d = Ls[j * TILE + j];
+        const double s = d > 0.0 ? Ls[j * TILE + r] / d : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cq + 16 * q;
+            if (c > j && r >= c) Ls[c * TILE + r] -= s * Ls[j * TILE + c];
         }
         __syncthreads();
     }
-    // inverse of the lower-triangular block: thread t owns column t
-    for (int i = 0; i < TILE; ++i) {
-        double x = 0.0;
-        if (i == t) x = 1.0 / Ls[i][i];
-        else if (i > t) {
-            double s = 0.0;
-            for (int k = t; k < i; ++k) s += Ls[i][k] * Xs[k][t];
-            x = -s / Ls[i][i];
+    double lv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = cq + 16 * q;
+        const double d = Ls[c * TILE + c];
+        if (!(d > 0.0) && r == 0) atomicMin(&bad, c + 1);  // first non-positive pivot
+        const double sd = d > 0.0 ? sqrt(d) : 1.0;
+        lv[q] = r > c ? Ls[c * TILE + r] / sd : (r == c ? sd : 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Ls[(cq + 16 * q) * TILE + r] = lv[q];
+    __syncthreads();
+    for (int i = 0; i < TILE - 1; ++i) {
+        const double f = r > i ? Ls[i * TILE + r] / Ls[i * TILE + i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cq + 16 * q;
+            if (r > i && c <= i) Bs[c * TILE + r] -= f * Bs[c * TILE + i];
         }
-        Xs[i][t] = x;
         __syncthreads();
     }
     const int blk = j0 / TILE;
-    for (int c = 0; c < TILE; ++c) {
-        A[(long)(j0 + t) + (long)ld * (j0 + c)] = t >= c ? Ls[t][c] : 0.0;
-        Dinv[(long)blk * TILE * TILE + t + (long)TILE * c] = Xs[t][c];
+    const double lrr = Ls[r * TILE + r];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = cq + 16 * q;
+        A[(long)(j0 + r) + (long)ld * (j0 + c)] = lv[q];
+        Dinv[(long)blk * TILE * TILE + r + (long)TILE * c] = r >= c ? Bs[c * TILE + r] / lrr : 0.0;
     }
+    if (t == 0 && bad <= TILE && *info == 0) *info = j0 + bad;
 }
 
 // zero the strictly upper part of the panel columns j0..j0+63 above the diagonal block
@@ -165,7 +306,7 @@ void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st)
     const int nb = ld / TILE;
     for (int b = 0; b < nb; ++b) {
         const int j0 = b * TILE;
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(64), 0, st, A, ld, j0, Dinv, info);
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, ld, j0, Dinv, info);
         if (j0 > 0) hipLaunchKernelGGL(zero_upper_panel_kernel, dim3(TILE), dim3(256), 0, st, A, ld, j0);
         const int rest = ld - j0 - TILE;
         if (rest <= 0) continue;
@@ -190,9 +331,13 @@ void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st)
     }
 }
 
-// L^-1 by blocked forward substitution over block rows:
-//   X[i, 0:i] = -Dinv_i * (L[i, 0:i] X[0:i, 0:i]),  X[i, i] = Dinv_i.
-// work: 64 x ld scratch.
+// L^-1 by recursive doubling over the 64 x 64 diagonal blocks (whose inverses
+// Dinv the Cholesky left behind).  A node of the block hierarchy splits as
+//   L = [L11 0; L21 L22]  ->  L^-1 = [X11 0; -X22 L21 X11  X22],
+// and every node of one level is independent, so a level is two batched
+// GEMMs (T = L21 X11 with X11 lower: k >= column; X21 = -X22 T with X22
+// lower: k <= row).  A level whose last node has a short right child runs
+// that node with its own sizes.  work: ld * ld / 2 doubles.
 __global__ void place_diag_blocks_kernel(double* X, int ld, const double* Dinv) {
     const int b = blockIdx.x;
     for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
@@ -204,29 +349,38 @@ __global__ void fill_zero_kernel(double* X, long n) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) X[t] = 0.0;
 }
 
+static void trtri_level(const double* L, double* X, int ld, double* work, int r0, int n1, int n2, int count,
+                        long stride, hipStream_t st) {
+    // T (n2 x n1) = L21 X11
+    Gemm g{};
+    g.M = n2; g.N = n1; g.K = n1;
+    g.A = L + (r0 + n1) + (long)ld * r0; g.lda = ld; g.strideA = stride;
+    g.B = X + r0 + (long)ld * r0; g.ldb = ld; g.strideB = stride;
+    g.C = work; g.ldc = n2; g.strideC = (long)n2 * n1;
+    g.alpha = 1.0; g.beta = 0.0; g.kmode = K_FROM_COLTILE; g.batch = count;
+    launch_gemm(g, st);
+    // X21 = -X22 T
+    Gemm h{};
+    h.M = n2; h.N = n1; h.K = n2;
+    h.A = X + (r0 + n1) + (long)ld * (r0 + n1); h.lda = ld; h.strideA = stride;
+    h.B = work; h.ldb = n2; h.strideB = (long)n2 * n1;
+    h.C = X + (r0 + n1) + (long)ld * r0; h.ldc = ld; h.strideC = stride;
+    h.alpha = -1.0; h.beta = 0.0; h.kmode = K_BELOW_ROWTILE; h.batch = count;
+    launch_gemm(h, st);
+}
+
 void launch_trtri_lower(const double* L, double* X, const double* Dinv, int ld, double* work, hipStream_t st) {
     const long n = (long)ld * ld;
     hipLaunchKernelGGL(fill_zero_kernel, dim3(4096), dim3(256), 0, st, X, n);
     const int nb = ld / TILE;
     hipLaunchKernelGGL(place_diag_blocks_kernel, dim3(nb), dim3(256), 0, st, X, ld, Dinv);
-    for (int b = 1; b < nb; ++b) {
-        const int i0 = b * TILE;
-        // Y = L[i0:i0+64, 0:i0] * X[0:i0, 0:i0]   (X lower: k >= j, i.e. rows of X from the column tile on)
-        Gemm g{};
-        g.M = TILE; g.N = i0; g.K = i0;
-        g.A = L + i0; g.lda = ld;
-        g.B = X; g.ldb = ld;
-        g.C = work; g.ldc = TILE;
-        g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
-        launch_gemm(g, st);
-        // X[i0:i0+64, 0:i0] = -Dinv_b * Y
-        Gemm h{};
-        h.M = TILE; h.N = i0; h.K = TILE;
-        h.A = Dinv + (long)b * TILE * TILE; h.lda = TILE;
-        h.B = work; h.ldb = TILE;
-        h.C = X + i0; h.ldc = ld;
-        h.alpha = -1.0; h.beta = 0.0; h.batch = 1;
-        launch_gemm(h, st);
+    for (int hb = 1; hb < nb; hb *= 2) {  // child size in blocks
+        const int b = hb * TILE;
+        const int full = nb / (2 * hb);
+        const long stride = 2L * b * (ld + 1);
+        if (full > 0) trtri_level(L, X, ld, work, 0, b, b, full, stride, st);
+        const int rem = nb - full * 2 * hb;
+        if (rem > hb) trtri_level(L, X, ld, work, full * 2 * b, b, (rem - hb) * TILE, 1, 0, st);
     }
 }
 

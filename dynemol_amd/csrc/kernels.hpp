@@ -28,7 +28,7 @@ int launch_build_SH(const BasisDev& b, int N, int ld, double* S, double* h, doub
                     hipStream_t st);
 
 // ---------------------------------------------------------------- dense FP64
-enum KMode { K_FULL = 0, K_BELOW_ROWTILE = 1, K_BELOW_COLTILE = 2, K_FROM_ROWTILE = 3 };
+enum KMode { K_FULL = 0, K_BELOW_ROWTILE = 1, K_BELOW_COLTILE = 2, K_FROM_ROWTILE = 3, K_FROM_COLTILE = 4 };
 
 struct Gemm {
     int M, N, K;
@@ -50,6 +50,7 @@ void launch_gemm(const Gemm& g, hipStream_t st);
 // matrix, plus L^-1 in Linv.  info (device int) gets the first failing
 // column + 1, or stays 0.  Dinv: (ld/64) blocks of 64x64 workspace.
 void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st);
+// work: ld * ld / 2 doubles
 void launch_trtri_lower(const double* L, double* Linv, const double* Dinv, int ld, double* work, hipStream_t st);
 void launch_symmetrize_from_lower(double* A, int ld, hipStream_t st);
 void launch_copy_matrix(const double* src, double* dst, long n, hipStream_t st);

@@ -60,7 +60,44 @@ def test_gemm_lower_only_and_kmodes():
                 assert np.array_equal(out[blk], C0[blk])
 
 
-@pytest.mark.parametrize("n", [64, 192, 448])
+@pytest.mark.parametrize("transA,transB", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_large_tiles(transA, transB):
+    """M, N >= 256 take the 128 x 128 pipelined kernel; ragged edges and odd K."""
+    rng = np.random.default_rng(3)
+    M, N, K = 300, 280, 70
+    _say(f"gemm128 M={M} N={N} K={K} tA={transA} tB={transB}")
+    A = rng.normal(size=(K, M) if transA else (M, K))
+    B = rng.normal(size=(N, K) if transB else (K, N))
+    C = rng.normal(size=(M, N))
+    out = hip.run_gemm(A, B, C, alpha=0.5, beta=-2.0, transA=transA, transB=transB)
+    ref = 0.5 * ((A.T if transA else A) @ (B.T if transB else B)) - 2.0 * C
+    assert np.abs(out - ref).max() < 1e-12
+
+
+def test_gemm_large_tiles_kmodes_and_lower_only():
+    rng = np.random.default_rng(4)
+    n = 448
+    _say("gemm128 kmodes")
+    L = np.tril(rng.normal(size=(n, n)))
+    B = rng.normal(size=(n, n))
+    assert np.abs(hip.run_gemm(L, B, np.zeros((n, n)), kmode=1) - L @ B).max() < 1e-11
+    assert np.abs(hip.run_gemm(B, L, np.zeros((n, n)), transB=1, kmode=2) - B @ L.T).max() < 1e-11
+    assert np.abs(hip.run_gemm(L, B, np.zeros((n, n)), transA=1, kmode=3) - L.T @ B).max() < 1e-11
+    # B lower: opB(k, j) = L[k, j] nonzero for k >= j
+    assert np.abs(hip.run_gemm(B, L, np.zeros((n, n)), kmode=4) - B @ L).max() < 1e-11
+    C0 = rng.normal(size=(n, n))
+    out = hip.run_gemm(B, B, C0, alpha=-1.0, beta=1.0, transB=1, lower_only=1)
+    ref = C0 - B @ B.T
+    for bi in range(n // 64):
+        for bj in range(n // 64):
+            blk = (slice(64 * bi, 64 * bi + 64), slice(64 * bj, 64 * bj + 64))
+            if bi >= bj:
+                assert np.abs(out[blk] - ref[blk]).max() < 1e-11
+            else:
+                assert np.array_equal(out[blk], C0[blk])
+
+
+@pytest.mark.parametrize("n", [64, 192, 448, 1024, 1216])
 def test_cholesky_and_inverse(n):
     rng = np.random.default_rng(n)
     _say(f"cholesky n={n}")
