@@ -180,13 +180,23 @@ extern "C" int dh_finalize(dh_context* c) {
     return DH_OK;
 }
 
+// Partial-sum buffer of a system: split-K variants (up to 16 RHS per pass) and
+// the symmetric two-phase variant for the RHS counts a system uses.
+static long partial_cplx_for(int N, int n_wp) {
+    const int ld = round_ld(N);
+    long n = (long)matvec_split_for(N, 1) * ld * 16;
+    const int rs[3] = {2 * n_wp, n_wp, 1};
+    for (int R : rs)
+        if (sym_supported(R, TRI_FULL)) n = std::max(n, sym_partial_cplx(ld, R));
+    return n;
+}
+
 extern "C" long long dh_system_bytes(int n_orb, int n_wp) {
     const long long ld = round_ld(n_orb);
     const long long R = 2LL * n_wp;
-    const int split = matvec_split_for(n_orb, 1);
     long long b = 3LL * ld * ld * 8;                       // Q1, Q2, H_orth
     b += (4 * R + 3LL * n_wp + n_wp) * ld * 16;             // phi, x0, x1, acc, out, hk
-    b += (long long)split * ld * 16 * 16;                  // partials
+    b += (long long)partial_cplx_for(n_orb, n_wp) * 16;    // partials
     b += 3LL * ld * 16 + (long long)KMAX_ALLOC * 24;      // Lanczos, Bessel
     return b;
 }
@@ -319,7 +329,7 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->out, (size_t)ld * 3 * n_wp);
     SALLOC(s->hk, (size_t)ld * n_wp);
     s->split_max = matvec_split_for(N, 1);
-    SALLOC(s->partial, (size_t)s->split_max * ld * 16);
+    SALLOC(s->partial, (size_t)partial_cplx_for(N, n_wp));
     SALLOC(s->counters, (size_t)ld);
     SALLOC(s->lv, (size_t)ld);
     SALLOC(s->lvp, (size_t)ld);
@@ -993,7 +1003,7 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     DALLOC(out, (size_t)ld * R);
     DALLOC(acc, (size_t)ld * R);
     DALLOC(coef, 8);
-    DALLOC(partial, (size_t)split * ld * 16);
+    DALLOC(partial, (size_t)std::max((long)split * ld * 16, sym_partial_cplx(ld, R <= 8 ? R : 8)));
     DALLOC(counters, (size_t)ld);
     DH_HIP(hipMemset(M, 0, sizeof(double) * (size_t)ld * ld));
     DH_HIP(hipMemset(x, 0, sizeof(cplx) * (size_t)ld * R));

@@ -771,9 +771,266 @@ void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, con
     }
 }
 
+// ------------------------------------------------------------------ symmetric half-storage term
+// H_orth is symmetric, so a term only needs the 64 x 64 tiles (I, J) with
+// J <= I: each off-diagonal tile is read once and applied twice, as H_IJ to
+// x_J (rows of block I) and as H_IJ^T to x_I (rows of block J).
+//
+// Phase 1 (sym_tiles_kernel): a workgroup takes a segment of S consecutive
+// tiles of block row I (S is picked so the grid fits the resident slots).
+//  * rows: lane l of wave w holds row i0+l for columns 16w..16w+15, read as
+//    coalesced 512 B column segments; x_J[j] is the same for the whole wave
+//    and comes in through scalar loads, and the row sums stay in registers
+//    over the segment;
+//  * columns: the tile goes through LDS; lane l of wave w then holds column
+//    j0+l for rows 16w..16w+15 (x_I again wave-uniform), and the four waves'
+//    column sums are added in LDS.
+// Column sums of tile (I, J) go to Pcol[I(I-1)/2 + J], row sums of segment s
+// to Prow[s] (64 x R complex each, in a.partial).
+// Phase 2 (sym_reduce_kernel): for block B, adds the segments of block row B
+// and Pcol[(I, B)] for I > B in a fixed order (deterministic), then runs the
+// Chebyshev epilogue.
+__host__ __device__ inline int sym_segments_in_row(int I, int S) { return (I + S) / S; }
+
+static int sym_segment_count(int nb, int S) {
+    int s = 0;
+    for (int I = 0; I < nb; ++I) s += sym_segments_in_row(I, S);
+    return s;
+}
+
+// shortest segment whose grid fits three resident workgroups per CU
+static int sym_seg_for(int nb) {
+    int S = 1;
+    while (sym_segment_count(nb, S) > 3 * 256) ++S;
+    return S;
+}
+
+long sym_partial_cplx(int ld, int R) {
+    const long nb = ld / TILE;
+    return (nb * (nb - 1) / 2 + sym_segment_count((int)nb, sym_seg_for((int)nb))) * TILE * R;
+}
+
+bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2 || R == 4 || R == 8); }
+
+__device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r, cplx y) {
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    if (a.epi == EPI_PLAIN) {
+        a.out[oi] = y;
+        return;
+    }
+    const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+    cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+    if (a.epi == EPI_CHEB) {
+        const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+        T.re -= x2.re;
+        T.im -= x2.im;
+        const cplx add = cmul(a.coef[a.k], T);
+        cplx acc = a.acc[oi];
+        acc.re += add.re;
+        acc.im += add.im;
+        a.acc[oi] = acc;
+    } else {
+        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+    a.out[oi] = T;
+}
+
+// x and H are read-only here and restrict-qualified, so wave-uniform x reads
+// become scalar loads.
+template <int R>
+__global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict__ Mg, const cplx* __restrict__ xg,
+                                                        cplx* __restrict__ part, int ld, int N, int x_stride,
+                                                        int x_off, int S) {
+    constexpr int HP = TILE + 1;  // padded LDS column
+    constexpr int XB = 16 / R;    // x rows per scalar-load batch (4R dwords each)
+    __shared__ double Hs[TILE * HP];
+    __shared__ double zred[4 * TILE * 2 * R];
+    const int nb = ld / TILE;
+    int seg = blockIdx.x, I = 0;
+    while (I < nb - 1 && seg >= sym_segments_in_row(I, S)) {
+        seg -= sym_segments_in_row(I, S);
+        ++I;
+    }
+    const int Ja = seg * S, Jb = min(I + 1, Ja + S);
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i0 = I * TILE, row = i0 + lane;
+    // H is read over the whole padded ld x ld tile.  Its padding is
+    // [0 0; 0 I] (from the identity-padded S and h, exactly), so H[i][j] = 0
+    // whenever one of i, j is < N and the other >= N: rows >= N of x never
+    // reach rows < N of the result, and they are read as row N - 1 (finite
+    // data) instead of the unwritten padding.  Results for rows >= N are
+    // discarded by phase 2.
+    double hn[16];
+    auto fetch = [&](int J) {
+        const int j0 = J * TILE + 16 * w;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) hn[s] = Mg[(long)(j0 + s) * ld + row];
+    };
+    auto xload = [&](int j, double* xr, double* xi) {  // wave-uniform row j of x
+        const cplx* p = xg + (long)min(j, N - 1) * x_stride + x_off;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const cplx v = p[q];
+            xr[q] = v.re;
+            xi[q] = v.im;
+        }
+    };
+    double yr[R], yi[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) yr[q] = yi[q] = 0.0;
+    fetch(Ja);
+    for (int J = Ja; J < Jb; ++J) {
+        const int j0 = J * TILE;
+        __syncthreads();  // previous tile's LDS readers are done
+        double hc[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            hc[s] = hn[s];
+            Hs[(16 * w + s) * HP + lane] = hn[s];
+        }
+        if (J + 1 < Jb) fetch(J + 1);
+        // rows of block I: H_IJ x_J over this wave's 16 columns.  Scalar loads
+        // return out of order (only a wait for all of them is possible), so
+        // they are issued in batches of XB rows (~64 SGPRs), then consumed.
+#pragma unroll
+        for (int s0 = 0; s0 < 16; s0 += XB) {
+            double xr[XB][R], xi[XB][R];
+#pragma unroll
+            for (int u = 0; u < XB; ++u) xload(j0 + 16 * w + s0 + u, xr[u], xi[u]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < XB; ++u)
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    yr[q] = fma(hc[s0 + u], xr[u][q], yr[q]);
+                    yi[q] = fma(hc[s0 + u], xi[u][q], yi[q]);
+                }
+#pragma unroll
+            for (int q = 0; q < R; ++q) asm volatile("" : "+v"(yr[q]), "+v"(yi[q]));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (J == I) continue;  // diagonal tile: the row sums cover it
+        __syncthreads();       // tile complete in LDS
+        // rows of block J: H_IJ^T x_I, column j0+lane over rows 16w..16w+15
+        double zr[R], zi[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) zr[q] = zi[q] = 0.0;
+#pragma unroll
+        for (int s0 = 0; s0 < 16; s0 += XB) {
+            double xr[XB][R], xi[XB][R], hv[XB];
+#pragma unroll
+            for (int u = 0; u < XB; ++u) {
+                hv[u] = Hs[lane * HP + 16 * w + s0 + u];
+                xload(i0 + 16 * w + s0 + u, xr[u], xi[u]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < XB; ++u)
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    zr[q] = fma(hv[u], xr[u][q], zr[q]);
+                    zi[q] = fma(hv[u], xi[u][q], zi[q]);
+                }
+#pragma unroll
+            for (int q = 0; q < R; ++q) asm volatile("" : "+v"(zr[q]), "+v"(zi[q]));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            zred[((w * TILE) + lane) * 2 * R + 2 * q] = zr[q];
+            zred[((w * TILE) + lane) * 2 * R + 2 * q + 1] = zi[q];
+        }
+        __syncthreads();
+        const long tI = (long)I * (I - 1) / 2 + J;
+        for (int e = t; e < TILE * R; e += 256) {
+            const int c = e / R, q = e % R;
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) {
+                sr += zred[(ww * TILE + c) * 2 * R + 2 * q];
+                si += zred[(ww * TILE + c) * 2 * R + 2 * q + 1];
+            }
+            part[tI * TILE * R + e] = cplx{sr, si};
+        }
+    }
+    // row sums: add the four waves' column groups in order
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        zred[(w * TILE + lane) * 2 * R + 2 * q] = yr[q];
+        zred[(w * TILE + lane) * 2 * R + 2 * q + 1] = yi[q];
+    }
+    __syncthreads();
+    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    for (int e = t; e < TILE * R; e += 256) {
+        const int rr = e / R, q = e % R;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            sr += zred[(ww * TILE + rr) * 2 * R + 2 * q];
+            si += zred[(ww * TILE + rr) * 2 * R + 2 * q + 1];
+        }
+        part[pcol + (long)blockIdx.x * TILE * R + e] = cplx{sr, si};
+    }
+}
+
+// Phase 2: workgroup (B, c) owns 64 consecutive complex outputs of block B
+// (rows B*64 + (c*64 + e) / R, column (c*64 + e) % R); 8 groups of 64
+// threads split the source list, added in group order.
+template <int R>
+__global__ __launch_bounds__(512) void sym_reduce_kernel(MatvecArgs a, int S) {
+    constexpr int G = 8;
+    __shared__ cplx red[G * 64];
+    const int nb = a.ld / TILE;
+    const int B = blockIdx.x, chunk = blockIdx.y;
+    const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
+    int seg0 = 0;
+    for (int I = 0; I < B; ++I) seg0 += sym_segments_in_row(I, S);
+    const int nseg = sym_segments_in_row(B, S);
+    const int L = nseg + (nb - 1 - B);   // sources: row segments, then Pcol(I, B), I > B
+    const int s0 = g * L / G, s1 = (g + 1) * L / G;
+    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    double sr = 0.0, si = 0.0;
+#pragma unroll 4
+    for (int src = s0; src < s1; ++src) {
+        long off;
+        if (src < nseg) {
+            off = pcol + (long)(seg0 + src) * TILE * R;
+        } else {
+            const long I = B + 1 + (src - nseg);
+            off = (I * (I - 1) / 2 + B) * TILE * R;
+        }
+        const cplx p = a.partial[off + e];
+        sr += p.re;
+        si += p.im;
+    }
+    red[t] = cplx{sr, si};
+    __syncthreads();
+    if (t >= 64) return;
+    cplx y{0.0, 0.0};
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+        y.re += red[gg * 64 + t].re;
+        y.im += red[gg * 64 + t].im;
+    }
+    const int i = B * TILE + e / R;
+    if (i < a.N) term_epilogue(a, i, e % R, y);
+}
+
+template <int R>
+static void launch_sym(const MatvecArgs& a, hipStream_t st) {
+    const int nb = a.ld / TILE;
+    const int S = sym_seg_for(nb);
+    hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
+                       a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
+    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(512), 0, st, a, S);
+}
+
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
 // counters); 1..4 = column-block kernels of different shapes.
-constexpr int MV_VARIANTS = 16;
+constexpr int MV_VARIANTS = 17;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -836,6 +1093,20 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
+    if (variant == 16) {
+        // symmetric half-storage two-phase kernel; other shapes use the column-block kernel
+        if (!sym_supported(a.R, a.tri)) {
+            launch_matvec_variant(a, 1, st);
+            return;
+        }
+        switch (a.R) {
+            case 1: launch_sym<1>(a, st); break;
+            case 2: launch_sym<2>(a, st); break;
+            case 4: launch_sym<4>(a, st); break;
+            default: launch_sym<8>(a, st); break;
+        }
+        return;
+    }
     if (variant == 15) {
         // matrix-core GEMM form; triangular shapes use the column-block kernel
         if (a.tri == TRI_FULL) launch_cheb_gemm(a, 0, st);

@@ -80,6 +80,9 @@ struct MatvecArgs {
     const float* M32;       // FP32 copy of M for the mixed-precision GEMM variant (may be null)
 };
 int matvec_split_for(int N, int R);
+// complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
+long sym_partial_cplx(int ld, int R);
+bool sym_supported(int R, int tri);
 void launch_matvec(const MatvecArgs& a, hipStream_t st);
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st);
 int matvec_variant_for(int N, int R, int tri);

@@ -119,8 +119,8 @@ def test_cholesky_reports_non_spd():
     assert info == 71
 
 
-@pytest.mark.parametrize("variant", list(range(16)))
-@pytest.mark.parametrize("n_wp", [1, 2, 3])
+@pytest.mark.parametrize("variant", list(range(17)))
+@pytest.mark.parametrize("n_wp", [1, 2, 3, 4])
 def test_matvec_variants_propagate_exactly(variant, n_wp):
     """Every matvec kernel variant the autotuner may pick gives the exact propagator."""
     rng = np.random.default_rng(10 + n_wp)
@@ -142,6 +142,38 @@ def test_matvec_variants_propagate_exactly(variant, n_wp):
         bras = [S @ k for k in kets]
         for w in range(n_wp):
             s.set_wavepacket(w, bras[w], kets[w])
+        s.propagate(0.0, dt)
+        for w in range(n_wp):
+            b, k = s.wavepacket(w)
+            ke = C @ (np.exp(-1j * E * dt / hip.H_BAR) * (C.T @ (S @ kets[w])))
+            assert np.abs(k - ke).max() <= 1e-10 * np.abs(ke).max()
+            assert np.abs(b - S @ ke).max() <= 1e-10 * np.abs(S @ ke).max()
+        s.close()
+        ctx.close()
+    finally:
+        hip.force_matvec_variant(-1)
+
+
+@pytest.mark.parametrize("n,n_wp", [(1100, 1), (1100, 2), (1100, 4)])
+def test_symmetric_half_storage_variant_large(n, n_wp):
+    """Variant 16 (lower-triangle tiles, two-phase) over many block rows and segments."""
+    rng = np.random.default_rng(n + n_wp)
+    _say(f"symmetric variant N={n} n_wp={n_wp}")
+    X = rng.normal(size=(n, n))
+    S = X @ X.T / n + np.eye(n)
+    A = rng.normal(size=(n, n))
+    h = (A + A.T) * 2.0
+    import scipy.linalg as sl
+    E, C = sl.eigh(h, S)
+    dt = 5e-4
+    hip.force_matvec_variant(16)
+    try:
+        ctx = hip.Context(0)
+        s = hip.System(ctx, None, n_wp=n_wp, n_frag=1, n_orb=n)
+        s.load_operators(S, h)
+        kets = [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(n_wp)]
+        for w in range(n_wp):
+            s.set_wavepacket(w, S @ kets[w], kets[w])
         s.propagate(0.0, dt)
         for w in range(n_wp):
             b, k = s.wavepacket(w)
