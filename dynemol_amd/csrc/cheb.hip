@@ -792,11 +792,13 @@ void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, con
 // Chebyshev epilogue.
 __host__ __device__ inline int sym_segments_in_row(int I, int S) { return (I + S) / S; }
 
-static int sym_segment_count(int nb, int S) {
-    int s = 0;
-    for (int I = 0; I < nb; ++I) s += sym_segments_in_row(I, S);
-    return s;
+// segments of block rows 0..B-1: sum_{k=1..B} ceil(k / S) in closed form
+__host__ __device__ inline int sym_segment_start(int B, int S) {
+    const int q = B / S, r = B - q * S;
+    return S * q * (q + 1) / 2 + r * (q + 1);
 }
+
+static int sym_segment_count(int nb, int S) { return sym_segment_start(nb, S); }
 
 // shortest segment whose grid fits three resident workgroups per CU
 static int sym_seg_for(int nb) {
@@ -847,12 +849,15 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
     __shared__ double Hs[TILE * HP];
     __shared__ double zred[4 * TILE * 2 * R];
     const int nb = ld / TILE;
-    int seg = blockIdx.x, I = 0;
-    while (I < nb - 1 && seg >= sym_segments_in_row(I, S)) {
-        seg -= sym_segments_in_row(I, S);
-        ++I;
+    // block row of this segment: the last I with sym_segment_start(I) <= blockIdx.x
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sym_segment_start(mid, S) <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
     }
-    const int Ja = seg * S, Jb = min(I + 1, Ja + S);
+    const int I = lo;
+    const int Ja = ((int)blockIdx.x - sym_segment_start(I, S)) * S, Jb = min(I + 1, Ja + S);
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int i0 = I * TILE, row = i0 + lane;
@@ -977,41 +982,47 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
 }
 
 // Phase 2: workgroup (B, c) owns 64 consecutive complex outputs of block B
-// (rows B*64 + (c*64 + e) / R, column (c*64 + e) % R); 8 groups of 64
-// threads split the source list, added in group order.
+// (rows B*64 + (c*64 + e) / R, column (c*64 + e) % R).  16 groups of 64
+// threads split the source list; each group issues up to SYM_MAXS loads
+// before adding them (latency, not bandwidth, bounds this kernel), and the
+// group sums are added in group order.
+constexpr int SYM_G = 16, SYM_MAXS = 8;
+
 template <int R>
-__global__ __launch_bounds__(512) void sym_reduce_kernel(MatvecArgs a, int S) {
-    constexpr int G = 8;
-    __shared__ cplx red[G * 64];
+__global__ __launch_bounds__(1024) void sym_reduce_kernel(MatvecArgs a, int S) {
+    __shared__ cplx red[SYM_G * 64];
     const int nb = a.ld / TILE;
     const int B = blockIdx.x, chunk = blockIdx.y;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
-    int seg0 = 0;
-    for (int I = 0; I < B; ++I) seg0 += sym_segments_in_row(I, S);
+    const int seg0 = sym_segment_start(B, S);
     const int nseg = sym_segments_in_row(B, S);
     const int L = nseg + (nb - 1 - B);   // sources: row segments, then Pcol(I, B), I > B
-    const int s0 = g * L / G, s1 = (g + 1) * L / G;
-    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    const int s0 = g * L / SYM_G, s1 = (g + 1) * L / SYM_G;
+    const long E = (long)TILE * R;
+    const long pcol = (long)nb * (nb - 1) / 2 * E;
     double sr = 0.0, si = 0.0;
-#pragma unroll 4
-    for (int src = s0; src < s1; ++src) {
-        long off;
-        if (src < nseg) {
-            off = pcol + (long)(seg0 + src) * TILE * R;
-        } else {
+    for (int base = s0; base < s1; base += SYM_MAXS) {
+        cplx p[SYM_MAXS];
+#pragma unroll
+        for (int u = 0; u < SYM_MAXS; ++u) {
+            const int src = min(base + u, s1 - 1);
             const long I = B + 1 + (src - nseg);
-            off = (I * (I - 1) / 2 + B) * TILE * R;
+            const long off = src < nseg ? pcol + (long)(seg0 + src) * E : (I * (I - 1) / 2 + B) * E;
+            p[u] = a.partial[off + e];
         }
-        const cplx p = a.partial[off + e];
-        sr += p.re;
-        si += p.im;
+#pragma unroll
+        for (int u = 0; u < SYM_MAXS; ++u)
+            if (base + u < s1) {
+                sr += p[u].re;
+                si += p[u].im;
+            }
     }
     red[t] = cplx{sr, si};
     __syncthreads();
     if (t >= 64) return;
     cplx y{0.0, 0.0};
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg) {
+    for (int gg = 0; gg < SYM_G; ++gg) {
         y.re += red[gg * 64 + t].re;
         y.im += red[gg * 64 + t].im;
     }
@@ -1025,7 +1036,7 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int S = sym_seg_for(nb);
     hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
                        a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
-    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(512), 0, st, a, S);
+    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(1024), 0, st, a, S);
 }
 
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
