@@ -26,9 +26,11 @@ DT_PS = 5.0e-4
 REBUILD_EVERY = 10
 JITTER = 0.02
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per term-kernel launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
-# (profiles/cheb_term_tio2_N4096_counters.txt); None until measured.
-MEASURED_TRAFFIC_BYTES = None
+# HBM bytes per Chebyshev term (both kernels of the symmetric variant, N=4096,
+# 4 RHS) from rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+# counter reads for the library's plain copy kernel over 256 MiB (FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000): profiles/cheb_term_sym_tio2_N4096_counters.md.
+MEASURED_TRAFFIC_BYTES = {16: int(((35409.8125 + 5487.5) * 2.0 + (10924.0 + 256.0)) * 1024)}
 
 
 def log(msg):
@@ -225,10 +227,20 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the library's plain copy kernel over MIB MiB (5 launches), the "
+                         "reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy > 0:
+        from dynemol_amd import hip
+        nbytes = args.calibrate_copy << 20
+        ms = hip.run_copy(nbytes, 5)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5,
+                          "ms_per_copy": ms, "GBs": 2 * nbytes / ms / 1e6}))
+        return
 
     from dynemol_amd.structures import tio2_slab_adsorbate
     st = tio2_slab_adsorbate(seed=2)
@@ -263,7 +275,7 @@ def main():
                                "hole HOMO), 0.5 fs steps, S/h/Cholesky/Lanczos rebuilt every 10 steps (jitter 0.02 A)",
                    "n_orbitals": N, "wavepackets": 2, "rebuild_every": REBUILD_EVERY},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_BYTES,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_BYTES.get(variant) if N == 4096 else None,
                      "kernel": f"Chebyshev term matvec, FP64, 4 RHS (autotuned variant {variant}: "
                                f"{'lower-triangle tiles of symmetric H' if variant == SYM_VARIANT else 'dense H'})",
                      "bytes_per_launch": bytes_term, "avg_launch_us": round(avg_launch_s * 1e6, 3),
