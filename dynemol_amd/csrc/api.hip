@@ -186,8 +186,10 @@ static long partial_cplx_for(int N, int n_wp) {
     const int ld = round_ld(N);
     long n = (long)matvec_split_for(N, 1) * ld * 16;
     const int rs[3] = {2 * n_wp, n_wp, 1};
-    for (int R : rs)
+    for (int R : rs) {
         if (sym_supported(R, TRI_FULL)) n = std::max(n, sym_partial_cplx(ld, R));
+        n = std::max(n, (gemm3_partial_doubles(N, ld, R) + 1) / 2);
+    }
     return n;
 }
 
@@ -706,7 +708,7 @@ static int chebyshev_series(dh_system* s) {
     a.M32 = s->H32;
     const bool mixed = s->precision == 1;
     auto term = [&](const MatvecArgs& args) {
-        if (mixed) launch_cheb_gemm(args, 1, st);
+        if (mixed) launch_gemm3(args, 1, reinterpret_cast<double*>(s->partial), st);
         else launch_matvec(args, st);
     };
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
@@ -1003,7 +1005,8 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     DALLOC(out, (size_t)ld * R);
     DALLOC(acc, (size_t)ld * R);
     DALLOC(coef, 8);
-    DALLOC(partial, (size_t)std::max((long)split * ld * 16, sym_partial_cplx(ld, R <= 8 ? R : 8)));
+    DALLOC(partial, (size_t)std::max(std::max((long)split * ld * 16, sym_partial_cplx(ld, R <= 8 ? R : 8)),
+                                     (gemm3_partial_doubles(N, ld, R) + 1) / 2));
     DALLOC(counters, (size_t)ld);
     DH_HIP(hipMemset(M, 0, sizeof(double) * (size_t)ld * ld));
     DH_HIP(hipMemset(x, 0, sizeof(cplx) * (size_t)ld * R));
@@ -1025,7 +1028,7 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
         a.M32 = M32;
     }
     auto run = [&]() {
-        if (variant == -1) launch_cheb_gemm(a, 1, nullptr);
+        if (variant == -1) launch_gemm3(a, 1, reinterpret_cast<double*>(partial), nullptr);
         else launch_matvec_variant(a, variant, nullptr);
     };
     for (int w = 0; w < 3; ++w) run();

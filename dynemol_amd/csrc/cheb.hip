@@ -701,6 +701,187 @@ __global__ __launch_bounds__(256) void cheb_gemm2_kernel(MatvecArgs a) {
     }
 }
 
+// Chebyshev epilogue for output row i, column r with y = (H x)_ir:
+//   EPI_PLAIN:       out = y
+//   EPI_CHEB_FIRST:  T = alpha y + beta x; acc = a_0 x + a_1 T; out = T
+//   EPI_CHEB:        T = alpha y + beta x - x2; acc += a_k T; out = T
+__device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r, cplx y) {
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    if (a.epi == EPI_PLAIN) {
+        a.out[oi] = y;
+        return;
+    }
+    const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
+    cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
+    if (a.epi == EPI_CHEB) {
+        const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+        T.re -= x2.re;
+        T.im -= x2.im;
+        const cplx add = cmul(a.coef[a.k], T);
+        cplx acc = a.acc[oi];
+        acc.re += add.re;
+        acc.im += add.im;
+        a.acc[oi] = acc;
+    } else {
+        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+    a.out[oi] = T;
+}
+
+// Split-K multi-RHS term GEMM: Y = H X with H N x N (FP64, or its FP32 copy
+// for PREC 1) and X the 2R real columns of the RHS.  A workgroup computes a
+// 128 (orbitals) x 128 (real columns) tile over one K slice; 4 waves in 2 x 2,
+// each a 64 x 64 quadrant of 4 x 4 MFMA 16x16x4 blocks.  16-deep k stages are
+// double-buffered in LDS and prefetched into registers.  The MFMA runs with
+// the operands swapped (D^T = X^T H^T), so a lane's results lie in 16
+// consecutive orbitals of one column.  Slice z writes its FP64 partial
+// product to P[z][c][i]; gemm3_reduce_kernel adds the slices in order and runs
+// the Chebyshev epilogue.  The FP32 path accumulates each K slice in FP32 and
+// adds the slices in FP64.
+constexpr int G3_T = 128;
+// k-stage depth: 16 (FP64) / 32 (FP32) keeps the double-buffered tiles at 74 KB of
+// LDS, so two workgroups share a CU and hide each other's barrier and load waits
+template <int PREC> struct G3 { static constexpr int BK = PREC == 0 ? 16 : 32; };
+// row pad: successive k-rows of an operand read start 32 banks apart (FP64) or
+// 16 banks apart (FP32), so the four rows a wave reads do not collide
+constexpr int G3_PAD = 16;
+
+int gemm3_splits(int N, int R) {
+    const int tiles = ((N + G3_T - 1) / G3_T) * ((2 * R + G3_T - 1) / G3_T);
+    int S = (512 + tiles - 1) / tiles;  // two workgroups per CU
+    const int kmax = (N + 255) / 256;  // at least 256 k per slice
+    if (S > kmax) S = kmax;
+    return S < 1 ? 1 : S;
+}
+
+long gemm3_partial_doubles(int N, int ld, int R) { return (long)gemm3_splits(N, R) * ld * 2 * R; }
+
+template <int PREC>
+__global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __restrict__ P, int KS) {
+    using T = typename std::conditional<PREC == 0, double, float>::type;
+    constexpr int G3_BK = G3<PREC>::BK;
+    __shared__ T As[2][G3_BK][G3_T + G3_PAD];
+    __shared__ T Bs[2][G3_BK][G3_T + G3_PAD];
+    const int N = a.N, C = 2 * a.R;
+    const int i0 = blockIdx.x * G3_T, c0 = blockIdx.y * G3_T, z = blockIdx.z;
+    const int klo = z * KS, khi = min(N, klo + KS);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wi = (w >> 1) * 64, wc = (w & 1) * 64;
+    const double* X = reinterpret_cast<const double*>(a.x);
+    const long xs = 2L * a.x_stride, xo = 2L * a.x_off;
+    using acc_t = typename std::conditional<PREC == 0, v4d, v4f>::type;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[p][q][v] = (T)0;
+    constexpr int LPT = G3_BK * G3_T / 256;  // elements of each tile per thread per stage
+    T ra[LPT], rb[LPT];
+    // H tile: rows i0..i0+127 (lane-contiguous), k0..k0+BK-1; X tile: k0..k0+BK-1, columns c0..c0+127
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < LPT; ++r) {
+            const int ii = t & 127, kk = (t >> 7) + 2 * r;
+            const int gi = i0 + ii, gk = k0 + kk;
+            T v = (T)0;
+            if (gi < N && gk < khi) {
+                if constexpr (PREC == 0) v = a.M[gi + (long)a.ld * gk];
+                else v = a.M32[gi + (long)a.ld * gk];
+            }
+            ra[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < LPT; ++r) {
+            const int cc = t & 127, kk = (t >> 7) + 2 * r;
+            const int gc = c0 + cc, gk = k0 + kk;
+            double v = 0.0;
+            if (gc < C && gk < khi) v = X[gk * xs + xo + gc];
+            rb[r] = (T)v;
+        }
+    };
+    auto swrite = [&](int s) {
+#pragma unroll
+        for (int r = 0; r < LPT; ++r) {
+            const int kk = (t >> 7) + 2 * r;
+            As[s][kk][t & 127] = ra[r];
+            Bs[s][kk][t & 127] = rb[r];
+        }
+    };
+    int s = 0;
+    if (klo < khi) {
+        gload(klo);
+        swrite(0);
+    }
+    __syncthreads();
+    for (int k0 = klo; k0 < khi; k0 += G3_BK) {
+        const bool more = k0 + G3_BK < khi;
+        if (more) gload(k0 + G3_BK);
+#pragma unroll
+        for (int ks = 0; ks < G3_BK; ks += 4) {
+            T av[4], bv[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) av[p] = As[s][ks + (lane >> 4)][wi + p * 16 + (lane & 15)];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bv[q] = Bs[s][ks + (lane >> 4)][wc + q * 16 + (lane & 15)];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if constexpr (PREC == 0)
+                        acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[q], av[p], acc[p][q], 0, 0, 0);
+                    else
+                        acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[q], av[p], acc[p][q], 0, 0, 0);
+                }
+        }
+        if (more) swrite(s ^ 1);
+        __syncthreads();
+        s ^= 1;
+    }
+    // D^T layout: f64 lane l, value v -> column (l >> 4) + 4v, orbital l & 15;
+    //             f32 lane l, value v -> column 4 (l >> 4) + v, orbital l & 15
+    double* Pz = P + (long)z * C * a.ld;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = i0 + wi + p * 16 + (lane & 15);
+                const int gc = c0 + wc + q * 16 + (PREC == 0 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
+                if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = (double)acc[p][q][v];
+            }
+}
+
+__global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const double* __restrict__ P, int S) {
+    const long n = (long)a.N * a.R;
+    const long e = blockIdx.x * 256L + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e / a.N), i = (int)(e % a.N);
+    const long C = 2L * a.R;
+    cplx y{0.0, 0.0};
+    for (int z = 0; z < S; ++z) {
+        const double* Pz = P + (long)z * C * a.ld;
+        y.re += Pz[(2L * r) * a.ld + i];
+        y.im += Pz[(2L * r + 1) * a.ld + i];
+    }
+    term_epilogue(a, i, r, y);
+}
+
+void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st) {
+    const int S = gemm3_splits(a.N, a.R);
+    const int BK = prec == 0 ? G3<0>::BK : G3<1>::BK;
+    const int KS = (((a.N + S - 1) / S) + BK - 1) / BK * BK;
+    const int Sr = (a.N + KS - 1) / KS;
+    dim3 grid((a.N + G3_T - 1) / G3_T, (2 * a.R + G3_T - 1) / G3_T, Sr);
+    if (prec == 0) hipLaunchKernelGGL((gemm3_kernel<0>), grid, dim3(256), 0, st, a, P, KS);
+    else hipLaunchKernelGGL((gemm3_kernel<1>), grid, dim3(256), 0, st, a, P, KS);
+    const long n = (long)a.N * a.R;
+    hipLaunchKernelGGL(gemm3_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, P, Sr);
+}
+
 static int g_gemm_bn = 0;  // 0: choose by size; 16 or 32 forces the tile width
 void set_gemm_tile(int bn) { g_gemm_bn = bn; }
 
@@ -814,29 +995,6 @@ long sym_partial_cplx(int ld, int R) {
 
 bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2 || R == 4 || R == 8); }
 
-__device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r, cplx y) {
-    const long oi = (long)i * a.out_stride + a.out_off + r;
-    if (a.epi == EPI_PLAIN) {
-        a.out[oi] = y;
-        return;
-    }
-    const cplx xv = a.x[(long)i * a.x_stride + a.x_off + r];
-    cplx T{a.alpha * y.re + a.beta * xv.re, a.alpha * y.im + a.beta * xv.im};
-    if (a.epi == EPI_CHEB) {
-        const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
-        T.re -= x2.re;
-        T.im -= x2.im;
-        const cplx add = cmul(a.coef[a.k], T);
-        cplx acc = a.acc[oi];
-        acc.re += add.re;
-        acc.im += add.im;
-        a.acc[oi] = acc;
-    } else {
-        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
-    }
-    a.out[oi] = T;
-}
 
 // x and H are read-only here and restrict-qualified, so wave-uniform x reads
 // become scalar loads.
@@ -1119,8 +1277,8 @@ void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
         return;
     }
     if (variant == 15) {
-        // matrix-core GEMM form; triangular shapes use the column-block kernel
-        if (a.tri == TRI_FULL) launch_cheb_gemm(a, 0, st);
+        // matrix-core split-K GEMM; triangular shapes use the column-block kernel
+        if (a.tri == TRI_FULL) launch_gemm3(a, 0, reinterpret_cast<double*>(a.partial), st);
         else launch_matvec_variant(a, 9, st);
         return;
     }

@@ -91,6 +91,10 @@ bool matvec_is_tuned(int N, int R, int tri);
 void matvec_force_variant(int v);
 int matvec_variant_count();
 void launch_cheb_gemm(const MatvecArgs& a, int prec, hipStream_t st);
+// split-K matrix-core term GEMM (prec 0: FP64, 1: FP32 products on M32); P holds
+// gemm3_partial_doubles(N, ld, R) doubles of partial products
+void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st);
+long gemm3_partial_doubles(int N, int ld, int R);
 // Grouped column-block matvec over many systems (R in {1, 2, 4, 8, 16}); each
 // task is (descriptor index, column block of grouped_cols_per_block(R) columns).
 int grouped_cols_per_block(int R);
