@@ -37,12 +37,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def initial_states(st, b, S_host=None):
+def device_S_h(st, b):
+    """S and h as the library builds them on the device (untimed setup)."""
+    from dynemol_amd import hip
+    with hip.Context(0) as ctx:
+        s = hip.System(ctx, b, n_wp=1, n_frag=len(st.fragment_names))
+        s.build(st.coords)
+        S, h = s.matrix(hip.DH_MAT_S), s.matrix(hip.DH_MAT_H)
+        s.close()
+    return S, h
+
+
+def initial_states(st, b):
     """LUMO/HOMO of the (h, S) pencil (whole system flagged El and Hl, FMO.f:294-350)."""
     import scipy.linalg as sl
-    from oracle import oracle as O  # setup only: generalised eigenvectors for the initial state
-    S = O.overlap(st.coords, b) if S_host is None else S_host
-    h = O.huckel(b, S)
+    S, h = device_S_h(st, b)
     nel = sum({"Ti": 4, "O": 6, "C": 4, "H": 1, "N": 5}[s] for s in st.symbols)
     homo = nel // 2 - 1
     E, C = sl.eigh(h, S, subset_by_index=[homo, homo + 1])
@@ -151,32 +160,29 @@ def run_batched(st, b, S, n_wp, steps, seed=11):
                           "terms": s.stats()["terms"]}
             s.close()
     err = float(np.abs(out[True]["pops"] - out[False]["pops"]).max())
+    log(f"batched: fp64 {out[False]['value']:.1f}, mixed {out[True]['value']:.1f} wavepacket-steps/s")
     return {"n_orbitals": b.N, "wavepackets": n_wp, "steps": steps + 1,
             "fp64_value": round(out[False]["value"], 3), "mixed_value": round(out[True]["value"], 3),
             "fp64_term_variant": out[False]["tuning"], "chebyshev_terms": out[False]["terms"],
             "mixed_pop_err_vs_fp64": err, "unit": "wavepacket-steps/s"}
 
 
-def run_serve(n_req, seed=5, lam=2.0, n_lo=256, n_hi=8192, budget_gib=96):
+def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96):
     """Poisson arrivals; N log-uniform, 1-16 wavepackets, 10-200 steps each (one context)."""
     from dynemol_amd import hip
     from dynemol_amd.serve import Request, Server, latency_stats
     from dynemol_amd.structures import heavy_cluster, log_uniform_int
-    from oracle import oracle as O  # request setup: S for S-normalised random initial states
     rng = np.random.default_rng(seed)
     reqs, t = [], 0.0
     for i in range(n_req):
         N = log_uniform_int(rng, n_lo, n_hi)
         st = heavy_cluster(int(rng.integers(1 << 30)), N)
         b = st.basis()
-        S = O.overlap(st.coords, b)
         w = int(rng.integers(1, 17))
-        kets = []
-        for _ in range(w):
-            k = rng.normal(size=N) + 1j * rng.normal(size=N)
-            kets.append(k / np.sqrt(np.real(np.conj(k) @ (S @ k))))
+        # AO coefficients; the server forms Psi_ket = C, Psi_bra = S C on the device at admission
+        ao = [(rng.normal(size=N) + 1j * rng.normal(size=N)) / np.sqrt(2.0 * N) for _ in range(w)]
         t += rng.exponential(1.0 / lam)
-        reqs.append(Request(rid=i, basis=b, coords=st.coords, psi_bra=[S @ k for k in kets], psi_ket=kets,
+        reqs.append(Request(rid=i, basis=b, coords=st.coords, psi_bra=None, psi_ket=ao, ao_init=True,
                             steps=int(rng.integers(10, 201)), arrival=t, n_frag=2))
     log(f"serve: {n_req} requests generated, N in [{min(r.basis.N for r in reqs)}, {max(r.basis.N for r in reqs)}]")
     with hip.Context(0) as ctx:
@@ -225,7 +231,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-wp", type=int, default=32)
     ap.add_argument("--batched-steps", type=int, default=5)
-    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=10)
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-sweep", type=str, default="",
+                    help="comma-separated arrival rates: run only the serving leg at each and print one JSON line per rate")
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB (5 launches), the "
@@ -234,6 +243,11 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.serve_sweep:
+        for lam in (float(x) for x in args.serve_sweep.split(",")):
+            log(f"serve sweep: rate {lam}/s")
+            print(json.dumps(run_serve(args.serve_requests, lam)), flush=True)
+        return
     if args.calibrate_copy > 0:
         from dynemol_amd import hip
         nbytes = args.calibrate_copy << 20
@@ -290,9 +304,10 @@ def main():
     if not args.no_extra:
         log("batched leg")
         out["batched"] = run_batched(st, b, S, args.batched_wp, args.batched_steps)
+        out["batched"]["fp64_vs_dropin"] = round(out["batched"]["fp64_value"] / value, 3)
         if args.serve_requests > 0:
             log("serve leg")
-            out["serve"] = run_serve(args.serve_requests)
+            out["serve"] = run_serve(args.serve_requests, args.serve_rate)
     if not args.no_cpu and args.cpu_steps > 0:
         log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(st, b, S, h, lumo, homo, args.cpu_steps)

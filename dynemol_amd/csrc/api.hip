@@ -76,6 +76,16 @@ struct dh_system {
     // [7] operator rebuilds, [8] rebuild wall ms
     double stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool timing = false;
+    // captured term series (hipGraph) for the two phi/acc buffer assignments;
+    // the key is everything the captured kernel arguments depend on
+    struct SeriesGraph {
+        hipGraphExec_t exec = nullptr;
+        int K = -1, prec = -1, variant = -1;
+        const void* phi = nullptr;
+        double c = 0.0, r = 0.0;
+    } graphs[2];
+    int next_graph = 0;
+    bool use_graph = true;
     bool tuned = false;
     int precision = 0;        // 0 = FP64, 1 = mixed (FP32 MFMA term GEMM)
     float* H32 = nullptr;     // FP32 copy of H_orth for the mixed mode
@@ -157,6 +167,8 @@ extern "C" int dh_system_destroy(dh_system* s) {
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
     if (s->H32) hipFree(s->H32);
+    for (auto& g : s->graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out,
@@ -330,6 +342,9 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->acc, (size_t)ld * R);
     SALLOC(s->out, (size_t)ld * 3 * n_wp);
     SALLOC(s->hk, (size_t)ld * n_wp);
+    // graphs save launch overhead on short terms; for large N a re-capture after
+    // every change of the spectral bounds costs more than it saves
+    s->use_graph = N <= 2048;
     s->split_max = matvec_split_for(N, 1);
     SALLOC(s->partial, (size_t)partial_cplx_for(N, n_wp));
     SALLOC(s->counters, (size_t)ld);
@@ -688,7 +703,7 @@ static int ensure_coefficients(dh_system* s, double dt) {
 }
 
 // One Chebyshev evaluation phi <- sum_k a_k T_k(Hh) phi, result in s->acc.
-static int chebyshev_series(dh_system* s) {
+static int series_launches(dh_system* s) {
     hipStream_t st = s->ctx->stream;
     const int R = 2 * s->W;
     const double c = s->coef_c, r = s->coef_r;
@@ -735,6 +750,54 @@ static int chebyshev_series(dh_system* s) {
         prev = outb;
     }
     DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+// The term series as one hipGraph launch: captured once per (K, bounds, buffer
+// assignment, precision, term kernel) and replayed on later steps.  The series
+// is kernels only (no memset nodes, no host synchronisation).
+static int chebyshev_series(dh_system* s) {
+    if (!s->use_graph) return series_launches(s);
+    hipStream_t st = s->ctx->stream;
+    const int variant = matvec_variant_for(s->N, 2 * s->W, TRI_FULL);
+    for (auto& g : s->graphs)
+        if (g.exec && g.K == s->K && g.phi == s->phi && g.c == s->coef_c && g.r == s->coef_r &&
+            g.prec == s->precision && g.variant == variant) {
+            DH_HIP(hipGraphLaunch(g.exec, st));
+            return DH_OK;
+        }
+    auto& g = s->graphs[s->next_graph];
+    s->next_graph ^= 1;
+    if (g.exec) {
+        DH_HIP(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+    }
+    DH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = series_launches(s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        dh::set_error("term series capture failed: %s", hipGetErrorString(e));
+        return DH_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        g.exec = nullptr;
+        dh::set_error("term series graph instantiation failed: %s", hipGetErrorString(ei));
+        return DH_ERR_HIP;
+    }
+    g.K = s->K;
+    g.phi = s->phi;
+    g.c = s->coef_c;
+    g.r = s->coef_r;
+    g.prec = s->precision;
+    g.variant = variant;
+    DH_HIP(hipGraphLaunch(g.exec, st));
     return DH_OK;
 }
 
@@ -851,6 +914,12 @@ extern "C" int dh_get_observables(dh_system* s, int wp, double* ao_bra, double* 
                               hipMemcpyDeviceToHost, st));
     if (erg) DH_HIP(hipMemcpyAsync(erg, s->erg + 2 * wp, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
     DH_HIP(hipStreamSynchronize(st));
+    return DH_OK;
+}
+
+extern "C" int dh_set_graph(dh_system* s, int on) {
+    CHECK_ARG(s, "dh_set_graph: null system");
+    s->use_graph = on != 0;
     return DH_OK;
 }
 

@@ -33,6 +33,7 @@ SYMBOLS = [
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
+    "dh_set_graph",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing"]
@@ -90,6 +91,7 @@ def lib():
     L.dh_test_cholesky.argtypes = [ctypes.c_int, _d, _d, _i]
     L.dh_test_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
+    L.dh_set_graph.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
     L.dh_set_precision.argtypes = [_vp, ctypes.c_int]
     L.dh_set_initial_state.argtypes = [_vp, ctypes.c_int, _d, _d]
@@ -275,6 +277,10 @@ class System:
 
     def set_timing(self, on=True):
         _check(lib().dh_set_timing(self.h, 1 if on else 0), "dh_set_timing")
+
+    def set_graph(self, on=True):
+        """Term series as one captured hipGraph per step (default) or as separate launches."""
+        _check(lib().dh_set_graph(self.h, 1 if on else 0), "dh_set_graph")
 
 
 def run_gemm(A, B, C, alpha=1.0, beta=0.0, transA=0, transB=0, kmode=0, lower_only=0):

@@ -8,8 +8,11 @@ dt): what an MD driver hands the propagator each nuclear step
     budget, building S, h and the orthogonal operators on the device at
     admission, then keeps the request's state resident until its k steps end;
   * advances every resident request one step per loop iteration, grouping FP64
-    requests by (padded) wavepacket count into one grouped launch per
-    Chebyshev term; mixed-precision requests run beside them on their own;
+    requests of N <= batch_max_n by (padded) wavepacket count into one grouped
+    launch per Chebyshev term (small systems: launch overhead dominates);
+    larger systems and mixed-precision requests step on their own with the
+    kernels the autotuner picked for their shapes (symmetric half-storage
+    matvec, matrix-core GEMM);
   * requests that arrive while others run join at the next iteration (no
     drain), and a finished request frees its slot and memory at once.
 
@@ -33,12 +36,13 @@ class Request:
     rid: int
     basis: object
     coords: np.ndarray
-    psi_bra: list            # per wavepacket, N complex
-    psi_ket: list
+    psi_bra: list            # per wavepacket, N complex (unused with ao_init)
+    psi_ket: list            # per wavepacket, N complex; AO coefficients C with ao_init
     steps: int
     dt: float = 5.0e-4
     n_frag: int = 1
     mixed: bool = False
+    ao_init: bool = False    # psi_ket holds C: Psi_ket = C, Psi_bra = S C formed on the device
     arrival: float = 0.0     # seconds on the server clock
     # filled by the server
     admitted: float = -1.0
@@ -61,8 +65,9 @@ def padded_wp(n):
 
 
 class Server:
-    def __init__(self, ctx: hip.Context, hbm_budget_bytes: int, max_active: int = 512):
+    def __init__(self, ctx: hip.Context, hbm_budget_bytes: int, max_active: int = 512, batch_max_n: int = 2048):
         self.ctx = ctx
+        self.batch_max_n = batch_max_n
         self.budget = int(hbm_budget_bytes)
         self.max_active = max_active
         self.used = 0
@@ -112,7 +117,10 @@ class Server:
             N = req.basis.N
             for i in range(w):
                 if i < len(req.psi_ket):
-                    s.set_wavepacket(i, req.psi_bra[i], req.psi_ket[i])
+                    if req.ao_init:
+                        s.set_initial_state(i, req.psi_ket[i], req.psi_ket[i])
+                    else:
+                        s.set_wavepacket(i, req.psi_bra[i], req.psi_ket[i])
                 else:  # padding wavepackets are zero and stay zero
                     z = np.zeros(N, dtype=np.complex128)
                     s.set_wavepacket(i, z, z)
@@ -123,10 +131,13 @@ class Server:
             admitted = True
         return admitted
 
+    def _solo(self, r):
+        return r.mixed or r.basis.N > self.batch_max_n
+
     def _groups(self):
         groups = {}
         for r in self.active:
-            if r.mixed:
+            if self._solo(r):
                 continue
             groups.setdefault((padded_wp(len(r.psi_ket)), r.dt), []).append(r)
         return groups
@@ -148,7 +159,7 @@ class Server:
                 r.t += dt
                 r.done_steps += 1
         for r in self.active:
-            if r.mixed:
+            if self._solo(r):
                 r.system.propagate(r.t, r.t + r.dt)
                 r.t += r.dt
                 r.done_steps += 1

@@ -119,6 +119,12 @@ int dh_get_stats(dh_system *sys, double *out9);
  * context stream that runs them); no counterpart in the reference, whose
  * only timing hook is the GPU_TIMING wall-clock print (GPU_Interface.cpp:33-49). */
 int dh_set_timing(dh_system *sys, int on);
+/* Run each step's Chebyshev term series as one captured hipGraph (on = 1;
+ * the default for N <= 2048) or as individual kernel launches (on = 0).  The graph is
+ * captured on the first step with given spectral bounds, term count and buffer
+ * assignment and replayed afterwards; results are identical either way.  No
+ * counterpart in the reference (one cuBLAS call per term, Taylor_gpu.cpp:295-330). */
+int dh_set_graph(dh_system *sys, int on);
 /* Matvec kernel variant chosen by the autotuner for this system's shapes:
  * [Chebyshev term (2*n_wp RHS), <H> (n_wp), L phi_bra, L^-T phi, L^T ket,
  * L^-1 bra, Lanczos (1 RHS)].  No counterpart in the reference (its GEMV tile
