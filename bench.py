@@ -196,6 +196,143 @@ def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
 
 
+def _exact(S, h, psi0, t):
+    """Exact propagation C exp(-i E t / hbar) C^T S psi0 from the (h, S) eigenpairs."""
+    import scipy.linalg as sl
+    from dynemol_amd import hip
+    E, C = sl.eigh(h, S)
+    return C @ (np.exp(-1j * E * t / hip.H_BAR) * (C.T @ (S @ psi0)))
+
+
+def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
+    """Many systems in one batch: C/H/N/O clusters, N log-uniform in [256, 2048],
+    4 wavepackets each (random S-normalised superpositions of 8 frontier orbitals),
+    grouped launches per Chebyshev term; sampled systems checked against the exact
+    propagator."""
+    import scipy.linalg as sl
+    from dynemol_amd import hip
+    from dynemol_amd.structures import heavy_cluster, log_uniform_int
+    rng = np.random.default_rng(seed)
+    out = {"systems": n_sys, "wavepackets_per_system": n_wp, "steps": steps}
+    with hip.Context(0) as ctx:
+        systems, init, mats = [], [], []
+        t0 = time.perf_counter()
+        for i in range(n_sys):
+            N = log_uniform_int(rng, 256, 2048)
+            st = heavy_cluster(int(rng.integers(1 << 30)), N)
+            b = st.basis()
+            s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
+            s.build(st.coords)
+            S, h = s.matrix(hip.DH_MAT_S), s.matrix(hip.DH_MAT_H)
+            mid = N // 2
+            E, C = sl.eigh(h, S, subset_by_index=[mid - 4, mid + 3])
+            cs = []
+            for w in range(n_wp):
+                c = C @ (rng.normal(size=8) + 1j * rng.normal(size=8))
+                c /= np.sqrt(np.real(np.conj(c) @ (S @ c)))
+                s.set_initial_state(w, c, c)
+                cs.append(c)
+            systems.append(s)
+            init.append(cs)
+            if (i + 1) % 32 == 0:
+                log(f"config3: {i + 1} systems built")
+            mats.append((S, h) if i < check else None)
+        out["setup_s"] = round(time.perf_counter() - t0, 2)
+        out["n_min"] = min(s.N for s in systems)
+        out["n_max"] = max(s.N for s in systems)
+        out["n_mean"] = round(float(np.mean([s.N for s in systems])), 1)
+        batch = hip.Batch(ctx, systems)
+        t = 0.0
+        batch.propagate(t, t + DT_PS)  # first step: includes the grouped task tables
+        t += DT_PS
+        t0 = time.perf_counter()
+        for step in range(steps - 1):
+            batch.propagate(t, t + DT_PS)
+            t += DT_PS
+            if (step + 2) % 100 == 0:
+                log(f"config3: step {step + 2}")
+        el = time.perf_counter() - t0
+        out["wavepacket_steps_per_s"] = round(n_sys * n_wp * (steps - 1) / el, 1)
+        out["ms_per_step"] = round(1e3 * el / (steps - 1), 3)
+        errs, perr = [], []
+        for i in range(check):
+            S, h = mats[i]
+            for w in range(n_wp):
+                ke = _exact(S, h, init[i][w], t)
+                _, k = systems[i].wavepacket(w)
+                errs.append(float(np.abs(k - ke).max() / np.abs(ke).max()))
+                _, pops, _ = systems[i].observables(w)
+                tot = float(np.real(np.conj(ke) @ (S @ ke)))
+                perr.append(abs(float(pops[-1]) - tot))
+        out["checked_systems"] = check
+        out["max_rel_psi_err_vs_exact"] = max(errs)
+        out["max_total_pop_err_vs_exact"] = max(perr)
+        batch.close()
+        for s in systems:
+            s.close()
+    return out
+
+
+def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
+    """One N = 8192 C/H/O interface system, n_wp random S-normalised wavepackets,
+    FP64 matrix-core terms against FP32 products (mixed): population, norm and
+    energy differences after each block of steps."""
+    from dynemol_amd import hip
+    from dynemol_amd.structures import interface_cluster
+    log("config4: generating the interface cluster")
+    st = interface_cluster(seed=seed)
+    b = st.basis()
+    log(f"config4: {st.n_atoms} atoms, N={b.N}")
+    rng = np.random.default_rng(seed)
+    out = {"n_orbitals": b.N, "atoms": st.n_atoms, "wavepackets": n_wp, "rhs_columns": 2 * n_wp, "steps": steps}
+    with hip.Context(0) as ctx:
+        res = {}
+        for mixed in (False, True):
+            s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
+            s.set_precision(mixed)
+            s.build(st.coords)
+            log(f"config4 {'mixed' if mixed else 'fp64'}: operators built, bounds {s.bounds()[:2]}")
+            if not mixed:
+                S = s.matrix(hip.DH_MAT_S)
+                kets = []
+                for _ in range(n_wp):
+                    k = rng.normal(size=b.N) + 1j * rng.normal(size=b.N)
+                    kets.append(k / np.sqrt(np.real(np.conj(k) @ (S @ k))))
+                bras = [S @ k for k in kets]
+                del S
+                log("config4: initial states ready")
+            for w in range(n_wp):
+                s.set_wavepacket(w, bras[w], kets[w])
+            t, el, hist = 0.0, 0.0, []
+            for step in range(steps):
+                t0 = time.perf_counter()
+                s.propagate(t, t + DT_PS)
+                el += time.perf_counter() - t0
+                t += DT_PS
+                if (step + 1) % 10 == 0:
+                    log(f"config4 {'mixed' if mixed else 'fp64'}: step {step + 1}, {s.stats()['terms']} terms")
+                if (step + 1) % check_every == 0:
+                    obs = [s.observables(w) for w in range(n_wp)]
+                    hist.append((np.array([o[1] for o in obs]), np.array([o[2] for o in obs])))
+            res[mixed] = {"hist": hist, "value": n_wp * steps / el, "terms": s.stats()["terms"],
+                          "bounds": s.bounds()[:2], "drift": s.stats()["norm_drift"]}
+            s.close()
+    f, m = res[False], res[True]
+    out["chebyshev_terms"] = f["terms"]
+    out["fp64_wavepacket_steps_per_s"] = round(f["value"], 2)
+    out["mixed_wavepacket_steps_per_s"] = round(m["value"], 2)
+    out["pop_err_by_step"] = {str(check_every * (i + 1)): float(np.abs(pm - pf).max())
+                              for i, ((pf, _), (pm, _)) in enumerate(zip(f["hist"], m["hist"]))}
+    pf, ef = f["hist"][-1]
+    pm, em = m["hist"][-1]
+    out["final_pop_err"] = float(np.abs(pm - pf).max())
+    out["final_norm_err"] = float(np.abs(pm[:, -1] - pf[:, -1]).max())
+    out["final_energy_err_eV"] = float(np.abs(em - ef).max())
+    out["fp64_norm_drift_last_step"] = f["drift"]
+    out["pop_err_bar_1e-6_met"] = out["final_pop_err"] <= 1e-6
+    return out
+
+
 def run_cpu_baseline(st, b, S, h, lumo, homo, steps):
     """Reference algorithm (Taylor.f Propagation, tolerance 1e-8) restated in C/OpenMP, timed on the host."""
     import scipy.linalg as sl
@@ -236,6 +373,10 @@ def main():
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated arrival rates: run only the serving leg at each and print one JSON line per rate")
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
+    ap.add_argument("--config3", type=int, default=0, metavar="STEPS",
+                    help="only run the 256-system batch case for STEPS steps and print its JSON line")
+    ap.add_argument("--config4", type=int, default=0, metavar="STEPS",
+                    help="only run the N=8192 FP64-vs-mixed precision case for STEPS steps and print its JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB (5 launches), the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -243,6 +384,12 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.config3:
+        print(json.dumps(run_config3(steps=args.config3)), flush=True)
+        return
+    if args.config4:
+        print(json.dumps(run_config4(steps=args.config4)), flush=True)
+        return
     if args.serve_sweep:
         for lam in (float(x) for x in args.serve_sweep.split(",")):
             log(f"serve sweep: rate {lam}/s")
@@ -254,6 +401,10 @@ def main():
         ms = hip.run_copy(nbytes, 5)
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5,
                           "ms_per_copy": ms, "GBs": 2 * nbytes / ms / 1e6}))
+        for mib in (32, 67, 134, 268, 1024):  # streaming-read ceiling at operator sizes
+            rms = hip.run_read(mib * 1000 * 1000, 20)
+            print(json.dumps({"read_bytes": mib * 1000 * 1000, "us_per_read": round(rms * 1e3, 2),
+                              "GBs": round(mib * 1000 * 1000 / rms / 1e6, 1)}))
         return
 
     from dynemol_amd.structures import tio2_slab_adsorbate

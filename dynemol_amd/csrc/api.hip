@@ -1046,6 +1046,52 @@ extern "C" int dh_test_copy(long long bytes, int repeats, double* ms_per_copy) {
     return DH_OK;
 }
 
+// Streaming read: each thread sums 16-byte loads over a grid-stride loop and
+// the block writes one partial (the read ceiling a matvec over H can approach).
+__global__ __launch_bounds__(256) void test_read_kernel(const double2* __restrict__ a, long n2, double* out) {
+    double s = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n2; i += (long)gridDim.x * 256L) {
+        const double2 v = a[i];
+        s += v.x + v.y;
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < 256; ++k) t += sh[k];
+        out[blockIdx.x] = t;
+    }
+}
+
+extern "C" int dh_test_read(long long bytes, int repeats, double* ms_per_read) {
+    CHECK_ARG(bytes >= 16 && repeats >= 1, "dh_test_read: bad argument");
+    const long n2 = (long)(bytes / 16);
+    const int grid = 2048;
+    double2* a = nullptr;
+    double* out = nullptr;
+    DALLOC(a, n2);
+    DALLOC(out, grid);
+    DH_HIP(hipMemset(a, 0, sizeof(double2) * n2));
+    hipEvent_t e0, e1;
+    DH_HIP(hipEventCreate(&e0));
+    DH_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(test_read_kernel, dim3(grid), dim3(256), 0, nullptr, a, n2, out);
+    DH_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(test_read_kernel, dim3(grid), dim3(256), 0, nullptr, a, n2, out);
+    DH_HIP(hipEventRecord(e1, nullptr));
+    DH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_read) *ms_per_read = ms / repeats;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(out);
+    return DH_OK;
+}
+
 extern "C" int dh_test_force_matvec_variant(int variant) {
     CHECK_ARG(variant >= -1 && variant < matvec_variant_count(), "dh_test_force_matvec_variant: bad variant");
     matvec_force_variant(variant);

@@ -729,6 +729,43 @@ __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r,
     a.out[oi] = T;
 }
 
+// The epilogue with its operands loaded ahead (they do not depend on y).
+struct EpiOperands {
+    cplx xv, x2, acc, ck;
+};
+
+__device__ __forceinline__ EpiOperands epilogue_operands(const MatvecArgs& a, int i, int r) {
+    EpiOperands o{};
+    if (a.epi == EPI_PLAIN) return o;
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    o.xv = a.x[(long)i * a.x_stride + a.x_off + r];
+    if (a.epi == EPI_CHEB) {
+        o.x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
+        o.acc = a.acc[oi];
+        o.ck = a.coef[a.k];
+    }
+    return o;
+}
+
+__device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, int r, cplx y, const EpiOperands& o) {
+    const long oi = (long)i * a.out_stride + a.out_off + r;
+    if (a.epi == EPI_PLAIN) {
+        a.out[oi] = y;
+        return;
+    }
+    cplx T{a.alpha * y.re + a.beta * o.xv.re, a.alpha * y.im + a.beta * o.xv.im};
+    if (a.epi == EPI_CHEB) {
+        T.re -= o.x2.re;
+        T.im -= o.x2.im;
+        const cplx add = cmul(o.ck, T);
+        a.acc[oi] = cplx{o.acc.re + add.re, o.acc.im + add.im};
+    } else {
+        const cplx t0 = cmul(a.coef[0], o.xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+    a.out[oi] = T;
+}
+
 // Split-K multi-RHS term GEMM: Y = H X with H N x N (FP64, or its FP32 copy
 // for PREC 1) and X the 2R real columns of the RHS.  A workgroup computes a
 // 128 (orbitals) x 128 (real columns) tile over one K slice; 4 waves in 2 x 2,
@@ -981,10 +1018,10 @@ __host__ __device__ inline int sym_segment_start(int B, int S) {
 
 static int sym_segment_count(int nb, int S) { return sym_segment_start(nb, S); }
 
-// shortest segment whose grid fits three resident workgroups per CU
+// shortest segment whose grid fits four resident workgroups per CU
 static int sym_seg_for(int nb) {
     int S = 1;
-    while (sym_segment_count(nb, S) > 3 * 256) ++S;
+    while (sym_segment_count(nb, S) > 4 * 256) ++S;
     return S;
 }
 
@@ -1004,8 +1041,11 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
                                                         int x_off, int S) {
     constexpr int HP = TILE + 1;  // padded LDS column
     constexpr int XB = 16 / R;    // x rows per scalar-load batch (4R dwords each)
+    // the four waves' column sums reuse the tile's LDS once it has been read:
+    // 33 KB per workgroup, four workgroups per CU
+    static_assert(4 * TILE * 2 * R <= TILE * HP, "column sums fit in the tile buffer");
     __shared__ double Hs[TILE * HP];
-    __shared__ double zred[4 * TILE * 2 * R];
+    double* const zred = Hs;
     const int nb = ld / TILE;
     // block row of this segment: the last I with sym_segment_start(I) <= blockIdx.x
     int lo = 0, hi = nb - 1;
@@ -1100,6 +1140,7 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
             for (int q = 0; q < R; ++q) asm volatile("" : "+v"(zr[q]), "+v"(zi[q]));
             __builtin_amdgcn_sched_barrier(0);
         }
+        __syncthreads();  // every wave is done with the tile
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             zred[((w * TILE) + lane) * 2 * R + 2 * q] = zr[q];
@@ -1158,6 +1199,12 @@ __global__ __launch_bounds__(1024) void sym_reduce_kernel(MatvecArgs a, int S) {
     const int s0 = g * L / SYM_G, s1 = (g + 1) * L / SYM_G;
     const long E = (long)TILE * R;
     const long pcol = (long)nb * (nb - 1) / 2 * E;
+    // the epilogue's operands do not depend on the partials: fetch them first
+    EpiOperands ep{};
+    if (t < 64) {
+        const int i = B * TILE + e / R;
+        if (i < a.N) ep = epilogue_operands(a, i, e % R);
+    }
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -1185,7 +1232,7 @@ __global__ __launch_bounds__(1024) void sym_reduce_kernel(MatvecArgs a, int S) {
         y.im += red[gg * 64 + t].im;
     }
     const int i = B * TILE + e / R;
-    if (i < a.N) term_epilogue(a, i, e % R, y);
+    if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
 }
 
 template <int R>

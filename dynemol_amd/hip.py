@@ -36,7 +36,7 @@ SYMBOLS = [
     "dh_set_graph",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
-                "dh_test_matvec_timing"]
+                "dh_test_matvec_timing", "dh_test_read"]
 
 # Every live context and system is released before interpreter shutdown, while
 # the HIP runtime is still up (a handle freed during finalisation aborts).
@@ -90,6 +90,7 @@ def lib():
                                ctypes.c_int, ctypes.c_int]
     L.dh_test_cholesky.argtypes = [ctypes.c_int, _d, _d, _i]
     L.dh_test_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
+    L.dh_test_read.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
     L.dh_set_graph.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
@@ -335,6 +336,12 @@ def time_matvec(N, R, variant, reps=20):
     us = ctypes.c_double(0.0)
     _check(lib().dh_test_matvec_timing(N, R, variant, reps, ctypes.byref(us)), "dh_test_matvec_timing")
     return us.value
+
+
+def run_read(nbytes, repeats=20):
+    ms = ctypes.c_double(0.0)
+    _check(lib().dh_test_read(int(nbytes), repeats, ctypes.byref(ms)), "dh_test_read")
+    return ms.value
 
 
 def run_copy(nbytes, repeats=20):

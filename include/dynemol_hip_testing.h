@@ -18,6 +18,9 @@ int dh_test_cholesky(int n, double *A, double *Linv, int *info);
 /* A plain device copy of `bytes` (read + write), repeated; for calibrating
  * memory-traffic counters against a kernel of known traffic. */
 int dh_test_copy(long long bytes, int repeats, double *ms_per_copy);
+/* A plain streaming read of `bytes` (16-byte loads, 2048 workgroups), repeated:
+ * the read bandwidth a kernel streaming an operator of that size can reach. */
+int dh_test_read(long long bytes, int repeats, double *ms_per_read);
 /* Average launch time (microseconds, HIP events) of one matvec variant on an
  * N x N FP64 operator with R complex right-hand sides (Chebyshev epilogue);
  * variant -1 times the FP32 matrix-core GEMM of the mixed mode. */
