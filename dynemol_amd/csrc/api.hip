@@ -53,6 +53,7 @@ struct dh_system {
     // propagation buffers
     cplx *phi = nullptr, *x0 = nullptr, *x1 = nullptr, *acc = nullptr, *out = nullptr, *hk = nullptr;
     cplx* partial = nullptr;
+    cplx* stage = nullptr;  // 2 x ld contiguous columns for host transfers
     unsigned* counters = nullptr;
     int split_max = 1;
     // Lanczos / Bessel
@@ -171,7 +172,7 @@ extern "C" int dh_system_destroy(dh_system* s) {
         if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
-    void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out,
+    void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out, s->stage,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->bounds, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
     for (void* p : ptrs)
@@ -281,7 +282,7 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     } while (0)
     // basis: one allocation, carved
     const int ns = (int)sh_n.size();
-    const size_t bytes = (size_t)n_atoms * 2 * sizeof(int) + (size_t)ns * 4 * sizeof(int) +
+    const size_t bytes = (size_t)n_atoms * 3 * sizeof(int) + (size_t)ns * 4 * sizeof(int) +
                          (size_t)ns * 4 * sizeof(double) + (size_t)N * 3 * sizeof(double) + 64 * 8;
     char* bm = nullptr;
     SALLOC(bm, bytes);
@@ -294,6 +295,7 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     };
     int* d_at_ns = (int*)carve(sizeof(int) * std::max(n_atoms, 1));
     int* d_at_s0 = (int*)carve(sizeof(int) * std::max(n_atoms, 1));
+    int* d_order = (int*)carve(sizeof(int) * std::max(n_atoms, 1));
     int* d_sh_n = (int*)carve(sizeof(int) * std::max(ns, 1));
     int* d_sh_l = (int*)carve(sizeof(int) * std::max(ns, 1));
     int* d_sh_nz = (int*)carve(sizeof(int) * std::max(ns, 1));
@@ -321,10 +323,24 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
         DH_HIP(hipMemcpy(d_ip, orb_ip, sizeof(double) * N, hipMemcpyHostToDevice));
         DH_HIP(hipMemcpy(d_kwh, orb_kwh, sizeof(double) * N, hipMemcpyHostToDevice));
         DH_HIP(hipMemcpy(d_vs, orb_vshift, sizeof(double) * N, hipMemcpyHostToDevice));
+        // atoms ordered by basis signature (shells' n, l, zetas): the S build maps
+        // a wavefront onto pairs of one element pair, so its lanes take the same branches
+        std::vector<int> order(n_atoms);
+        for (int a = 0; a < n_atoms; ++a) order[a] = a;
+        auto key = [&](int a) {
+            std::vector<double> k;
+            for (int q = 0; q < at_ns[a]; ++q) {
+                const int sh = at_s0[a] + q;
+                k.insert(k.end(), {(double)sh_n[sh], (double)sh_l[sh], sh_z[2 * sh], sh_z[2 * sh + 1]});
+            }
+            return k;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+        DH_HIP(hipMemcpy(d_order, order.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
         SALLOC(s->coords, (size_t)n_atoms * 3);
     }
     s->bd = BasisDev{n_atoms, s->coords, d_at_ns, d_at_s0, d_sh_n, d_sh_l, d_sh_nz, d_sh_o, d_sh_z, d_sh_c,
-                     d_ip, d_kwh, d_vs};
+                     d_ip, d_kwh, d_vs, d_order};
     SALLOC(s->frag, N);
     {
         std::vector<int> fr(N, 0);
@@ -342,6 +358,7 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->acc, (size_t)ld * R);
     SALLOC(s->out, (size_t)ld * 3 * n_wp);
     SALLOC(s->hk, (size_t)ld * n_wp);
+    SALLOC(s->stage, (size_t)ld * 2);
     // graphs save launch overhead on short terms; for large N a re-capture after
     // every change of the spectral bounds costs more than it saves
     s->use_graph = N <= 2048;
@@ -595,6 +612,38 @@ extern "C" int dh_build_operators(dh_system* s, double cutoff, double s_drop) {
     return dh_orthogonalize(s, 0);
 }
 
+// Host <-> device transfers as contiguous copies: a strided host<->device copy
+// with small rows goes row by row through pageable staging, which is slow at
+// large N.  Complex columns pass through the system's staging columns and a
+// strided copy kernel; matrices are copied whole when the pitches agree and
+// column by column otherwise.
+static int put_column(dh_system* s, cplx* dst, long dst_stride, const double* host, int slot) {
+    hipStream_t st = s->ctx->stream;
+    cplx* stg = s->stage + (size_t)slot * s->ld;
+    DH_HIP(hipMemcpyAsync(stg, host, sizeof(cplx) * s->N, hipMemcpyHostToDevice, st));
+    launch_strided_copy(dst, dst_stride, stg, 1, s->N, st);
+    return DH_OK;
+}
+
+static int get_column(dh_system* s, double* host, const cplx* src, long src_stride, int slot) {
+    hipStream_t st = s->ctx->stream;
+    cplx* stg = s->stage + (size_t)slot * s->ld;
+    launch_strided_copy(stg, 1, src, src_stride, s->N, st);
+    DH_HIP(hipMemcpyAsync(host, stg, sizeof(cplx) * s->N, hipMemcpyDeviceToHost, st));
+    return DH_OK;
+}
+
+static int copy_matrix(double* dst, long ldd, const double* src, long lds, int n, hipMemcpyKind kind,
+                       hipStream_t st) {
+    if (ldd == lds) {
+        DH_HIP(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)ldd * n, kind, st));
+        return DH_OK;
+    }
+    for (int j = 0; j < n; ++j)
+        DH_HIP(hipMemcpyAsync(dst + (size_t)ldd * j, src + (size_t)lds * j, sizeof(double) * n, kind, st));
+    return DH_OK;
+}
+
 extern "C" int dh_load_operators(dh_system* s, const double* S, const double* h, int ldh) {
     CHECK_ARG(s && S && h, "dh_load_operators: null argument");
     CHECK_ARG(ldh >= s->N, "dh_load_operators: ld < N");
@@ -604,10 +653,16 @@ extern "C" int dh_load_operators(dh_system* s, const double* S, const double* h,
     hipStream_t st = s->ctx->stream;
     s->ctx->ws_owner = s;
     launch_identity_pad(s->ctx->S, s->ctx->h, s->N, s->ld, st);
-    DH_HIP(hipMemcpy2DAsync(s->ctx->S, sizeof(double) * s->ld, S, sizeof(double) * ldh, sizeof(double) * s->N, s->N,
-                            hipMemcpyHostToDevice, st));
-    DH_HIP(hipMemcpy2DAsync(s->ctx->h, sizeof(double) * s->ld, h, sizeof(double) * ldh, sizeof(double) * s->N, s->N,
-                            hipMemcpyHostToDevice, st));
+    // N x N only: rows N..ld-1 keep the identity padding (a whole-column copy
+    // would bring in the caller's padding rows)
+    const bool whole = ldh == s->N && s->ld == s->N;
+    for (int j = 0; j < (whole ? 1 : s->N); ++j) {
+        const size_t cnt = whole ? (size_t)s->N * s->N : (size_t)s->N;
+        DH_HIP(hipMemcpyAsync(s->ctx->S + (size_t)s->ld * j, S + (size_t)ldh * j, sizeof(double) * cnt,
+                              hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(s->ctx->h + (size_t)s->ld * j, h + (size_t)ldh * j, sizeof(double) * cnt,
+                              hipMemcpyHostToDevice, st));
+    }
     return DH_OK;
 }
 
@@ -629,8 +684,8 @@ extern "C" int dh_get_matrix(dh_system* s, int which, double* out, int ldo) {
         dh::set_error("dh_get_matrix: matrix %d not available", which);
         return DH_ERR_STATE;
     }
-    DH_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, src, sizeof(double) * s->ld, sizeof(double) * s->N, s->N,
-                            hipMemcpyDeviceToHost, s->ctx->stream));
+    int rc = copy_matrix(out, ldo, src, s->ld, s->N, hipMemcpyDeviceToHost, s->ctx->stream);
+    if (rc) return rc;
     DH_HIP(hipStreamSynchronize(s->ctx->stream));
     return DH_OK;
 }
@@ -656,10 +711,11 @@ extern "C" int dh_set_wavepacket(dh_system* s, int wp, const double* bra, const 
     DH_HIP(hipSetDevice(s->ctx->device));
     const int W = s->W;
     hipStream_t st = s->ctx->stream;
-    DH_HIP(hipMemcpy2DAsync(s->out + wp, sizeof(cplx) * 3 * W, bra, sizeof(cplx), sizeof(cplx), s->N,
-                            hipMemcpyHostToDevice, st));
-    DH_HIP(hipMemcpy2DAsync(s->out + 2 * W + wp, sizeof(cplx) * 3 * W, ket, sizeof(cplx), sizeof(cplx), s->N,
-                            hipMemcpyHostToDevice, st));
+    int rc = put_column(s, s->out + wp, 3 * W, bra, 0);
+    if (rc) return rc;
+    rc = put_column(s, s->out + 2 * W + wp, 3 * W, ket, 1);
+    if (rc) return rc;
+    DH_HIP(hipGetLastError());
     DH_HIP(hipStreamSynchronize(st));
     s->phi_dirty = true;
     s->state_set = true;
@@ -671,12 +727,10 @@ extern "C" int dh_get_wavepacket(dh_system* s, int wp, double* bra, double* ket)
     DH_HIP(hipSetDevice(s->ctx->device));
     const int W = s->W;
     hipStream_t st = s->ctx->stream;
-    if (bra)
-        DH_HIP(hipMemcpy2DAsync(bra, sizeof(cplx), s->out + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
-                                hipMemcpyDeviceToHost, st));
-    if (ket)
-        DH_HIP(hipMemcpy2DAsync(ket, sizeof(cplx), s->out + 2 * W + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
-                                hipMemcpyDeviceToHost, st));
+    int rc = DH_OK;
+    if (bra && (rc = get_column(s, bra, s->out + wp, 3 * W, 0))) return rc;
+    if (ket && (rc = get_column(s, ket, s->out + 2 * W + wp, 3 * W, 1))) return rc;
+    DH_HIP(hipGetLastError());
     DH_HIP(hipStreamSynchronize(st));
     return DH_OK;
 }
@@ -906,9 +960,10 @@ extern "C" int dh_get_observables(dh_system* s, int wp, double* ao_bra, double* 
     DH_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     const int W = s->W;
-    if (ao_bra)
-        DH_HIP(hipMemcpy2DAsync(ao_bra, sizeof(cplx), s->out + W + wp, sizeof(cplx) * 3 * W, sizeof(cplx), s->N,
-                                hipMemcpyDeviceToHost, st));
+    if (ao_bra) {
+        const int rc = get_column(s, ao_bra, s->out + W + wp, 3 * W, 0);
+        if (rc) return rc;
+    }
     if (pops)
         DH_HIP(hipMemcpyAsync(pops, s->pops + (size_t)wp * (s->n_frag + 1), sizeof(double) * (s->n_frag + 1),
                               hipMemcpyDeviceToHost, st));
@@ -1430,8 +1485,10 @@ extern "C" int dh_set_initial_state(dh_system* s, int wp, const double* ao_bra, 
     s->phi_dirty = false;
     // stage C_bra, C_ket in the hk scratch (N x W is enough for one column pair when W >= 2; use x1 otherwise)
     cplx* stage = s->x1;  // N x R scratch
-    DH_HIP(hipMemcpy2DAsync(stage + 0, sizeof(cplx) * R, ao_bra, sizeof(cplx), sizeof(cplx), s->N, hipMemcpyHostToDevice, st));
-    DH_HIP(hipMemcpy2DAsync(stage + 1, sizeof(cplx) * R, ao_ket, sizeof(cplx), sizeof(cplx), s->N, hipMemcpyHostToDevice, st));
+    int prc = put_column(s, stage + 0, R, ao_bra, 0);
+    if (prc) return prc;
+    prc = put_column(s, stage + 1, R, ao_ket, 1);
+    if (prc) return prc;
     MatvecArgs a{};
     a.ld = s->ld; a.N = s->N; a.partial = s->partial; a.counters = s->counters; a.epi = EPI_PLAIN;
     a.M = s->Q1; a.tri = TRI_LOWER; a.R = 1; a.split = matvec_split_for(s->N, 1);

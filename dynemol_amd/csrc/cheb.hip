@@ -989,6 +989,17 @@ void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, con
     }
 }
 
+// One strided complex column to another (host transfers go through a contiguous
+// staging column and this copy).
+__global__ __launch_bounds__(256) void strided_copy_kernel(cplx* dst, long ds, const cplx* src, long ss, int n) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < n) dst[i * ds] = src[i * ss];
+}
+
+void launch_strided_copy(cplx* dst, long ds, const cplx* src, long ss, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(strided_copy_kernel, dim3((n + 255) / 256), dim3(256), 0, st, dst, ds, src, ss, n);
+}
+
 // ------------------------------------------------------------------ symmetric half-storage term
 // H_orth is symmetric, so a term only needs the 64 x 64 tiles (I, J) with
 // J <= I: each off-diagonal tile is read once and applied twice, as H_IJ to

@@ -22,6 +22,7 @@ struct BasisDev {
     const double* ip;         // [N] per orbital
     const double* kwh;        // [N]
     const double* vshift;     // [N]
+    const int* atom_order;    // [n_atoms] atoms sorted by basis signature (groups equal elements)
 };
 
 int launch_build_SH(const BasisDev& b, int N, int ld, double* S, double* h, double cutoff, double s_drop,
@@ -82,6 +83,7 @@ struct MatvecArgs {
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
 long sym_partial_cplx(int ld, int R);
+void launch_strided_copy(cplx* dst, long ds, const cplx* src, long ss, int n, hipStream_t st);
 bool sym_supported(int R, int tri);
 void launch_matvec(const MatvecArgs& a, hipStream_t st);
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st);

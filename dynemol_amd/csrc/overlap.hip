@@ -42,7 +42,9 @@ __device__ void solap_dev(int na, int la, double za, int nb, int lb, double zb, 
 #define SM(n, np) smat[(n) * SQ_D + (np)]
 #define SA(n, np) smataux[(n) * SQ_D + (np)]
 #define SMM(n, np) smatm[(n) * SQ_D + (np)]
-    for (int i = 0; i < SQ_HA * SQ_HG; ++i) h1f1[i] = 0.0;
+    // h1f1 needs no clearing: level ngmax is filled for a = 1..namax, level j
+    // for a = 1..min(namax, j - 1) from level j + 1, and every later read
+    // (j + 1, k + 2) has j + 1 <= min(namax, k + 1).
     for (int i = 0; i < SQ_D * SQ_D; ++i) {
         smat[i] = 0.0;
         smataux[i] = 0.0;
@@ -336,17 +338,25 @@ __global__ void init_SH_kernel(double* S, double* h, int N, int ld, const double
 }
 
 __global__ void build_SH_kernel(const BasisDev b, double* S, double* h, int ld, double cutoff, double s_drop) {
+    // Shell-pair major, then pairs (p > r) of atoms in basis-signature order:
+    // the 64 lanes of a wavefront share one shell pair and mostly one element
+    // pair, so they follow the same SOLAP branches.
     const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long npair = (long)b.n_atoms * (b.n_atoms - 1) / 2;
-    const int sp = (int)(tid % (MAX_SHELLS * MAX_SHELLS));
-    const long q = tid / (MAX_SHELLS * MAX_SHELLS);
-    if (q >= npair) return;
-    // q -> (ia > ib): q = ia*(ia-1)/2 + ib
-    int ia = (int)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
-    while ((long)ia * (ia - 1) / 2 > q) --ia;
-    while ((long)(ia + 1) * ia / 2 <= q) ++ia;
-    const int ib = (int)(q - (long)ia * (ia - 1) / 2);
-    const int sa = sp / MAX_SHELLS, sb = sp % MAX_SHELLS;
+    const int sp = (int)(tid / npair);
+    const long q = tid - (long)sp * npair;
+    if (sp >= MAX_SHELLS * MAX_SHELLS) return;
+    // q -> (p > r): q = p*(p-1)/2 + r
+    int p = (int)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
+    while ((long)p * (p - 1) / 2 > q) --p;
+    while ((long)(p + 1) * p / 2 <= q) ++p;
+    const int r = (int)(q - (long)p * (p - 1) / 2);
+    const int ap = b.atom_order[p], ar = b.atom_order[r];
+    // the reference's roles: A is the atom of higher index (overlap_D.f:144-311)
+    const bool p_is_a = ap > ar;
+    const int ia = p_is_a ? ap : ar, ib = p_is_a ? ar : ap;
+    const int sa = p_is_a ? sp / MAX_SHELLS : sp % MAX_SHELLS;
+    const int sb = p_is_a ? sp % MAX_SHELLS : sp / MAX_SHELLS;
     if (sa >= b.atom_nshell[ia] || sb >= b.atom_nshell[ib]) return;
     const double xab = b.coords[3 * ib + 0] - b.coords[3 * ia + 0];
     const double yab = b.coords[3 * ib + 1] - b.coords[3 * ia + 1];
