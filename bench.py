@@ -374,6 +374,9 @@ def main():
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated arrival rates: run only the serving leg at each and print one JSON line per rate")
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
+    ap.add_argument("--term-variant", type=int, default=-1,
+                    help="force one matvec variant instead of the autotuner's choice (for profiling: a "
+                         "kernel tracer adds per-dispatch cost that skews the autotuner's timings)")
     ap.add_argument("--config3", type=int, default=0, metavar="STEPS",
                     help="only run the 256-system batch case for STEPS steps and print its JSON line")
     ap.add_argument("--config4", type=int, default=0, metavar="STEPS",
@@ -408,6 +411,9 @@ def main():
                               "GBs": round(mib * 1000 * 1000 / rms / 1e6, 1)}))
         return
 
+    if args.term_variant >= 0:
+        from dynemol_amd import hip
+        hip.force_matvec_variant(args.term_variant)
     from dynemol_amd.structures import tio2_slab_adsorbate
     st = tio2_slab_adsorbate(seed=2)
     b = st.basis()

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
 constexpr int G2_T = 128, G2_BK = 16, G2_PAD = G2_T + 4;
 
 template <int TA, int TB>
-__global__ __launch_bounds__(256) void gemm128_f64_kernel(Gemm g) {
+__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(Gemm g) {
     __shared__ double As[2][G2_BK][G2_PAD];
     __shared__ double Bs[2][G2_BK][G2_PAD];
     const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
