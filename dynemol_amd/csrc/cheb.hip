@@ -1197,9 +1197,11 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
 // before adding them (latency, not bandwidth, bounds this kernel), and the
 // group sums are added in group order.
 constexpr int SYM_G = 16, SYM_MAXS = 8;
+constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per source group
+static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
 
 template <int R>
-__global__ __launch_bounds__(1024) void sym_reduce_kernel(MatvecArgs a, int S) {
+__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecArgs a, int S) {
     __shared__ cplx red[SYM_G * 64];
     const int nb = a.ld / TILE;
     const int B = blockIdx.x, chunk = blockIdx.y;
@@ -1252,7 +1254,7 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int S = sym_seg_for(nb);
     hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
                        a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
-    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(1024), 0, st, a, S);
+    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
 
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,

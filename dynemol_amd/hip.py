@@ -17,7 +17,8 @@ import weakref
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdynemol_hip.so")
+# DYNEMOL_HIP_LIB points the binding at another build of the same library
+LIB_PATH = os.environ.get("DYNEMOL_HIP_LIB", os.path.join(HERE, "libdynemol_hip.so"))
 
 DH_MAT_S, DH_MAT_H, DH_MAT_L, DH_MAT_LINV, DH_MAT_HORTH, DH_MAT_HPRIME = range(6)
 H_BAR = 6.58264e-4
