@@ -276,8 +276,9 @@ def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
 
 def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
     """One N = 8192 C/H/O interface system, n_wp random S-normalised wavepackets,
-    FP64 matrix-core terms against FP32 products (mixed): population, norm and
-    energy differences after each block of steps."""
+    FP64 matrix-core terms against FP32 products (mixed) and against the
+    FP32-stored operator with FP64 products: population, norm and energy
+    differences after each block of steps."""
     from dynemol_amd import hip
     from dynemol_amd.structures import interface_cluster
     log("config4: generating the interface cluster")
@@ -288,12 +289,13 @@ def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
     out = {"n_orbitals": b.N, "atoms": st.n_atoms, "wavepackets": n_wp, "rhs_columns": 2 * n_wp, "steps": steps}
     with hip.Context(0) as ctx:
         res = {}
-        for mixed in (False, True):
+        names = {hip.DH_PREC_FP64: "fp64", hip.DH_PREC_MIXED: "mixed", hip.DH_PREC_F32_STORAGE: "f32_storage"}
+        for mode, name in names.items():
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
-            s.set_precision(mixed)
+            s.set_precision(mode)
             s.build(st.coords)
-            log(f"config4 {'mixed' if mixed else 'fp64'}: operators built, bounds {s.bounds()[:2]}")
-            if not mixed:
+            log(f"config4 {name}: operators built, bounds {s.bounds()[:2]}")
+            if mode == hip.DH_PREC_FP64:
                 S = s.matrix(hip.DH_MAT_S)
                 K = rng.normal(size=(b.N, n_wp)) + 1j * rng.normal(size=(b.N, n_wp))
                 SK = S @ np.ascontiguousarray(K.real) + 1j * (S @ np.ascontiguousarray(K.imag))  # real BLAS GEMMs
@@ -311,26 +313,28 @@ def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
                 el += time.perf_counter() - t0
                 t += DT_PS
                 if (step + 1) % 10 == 0:
-                    log(f"config4 {'mixed' if mixed else 'fp64'}: step {step + 1}, {s.stats()['terms']} terms")
+                    log(f"config4 {name}: step {step + 1}, {s.stats()['terms']} terms")
                 if (step + 1) % check_every == 0:
                     obs = [s.observables(w) for w in range(n_wp)]
                     hist.append((np.array([o[1] for o in obs]), np.array([o[2] for o in obs])))
-            res[mixed] = {"hist": hist, "value": n_wp * steps / el, "terms": s.stats()["terms"],
-                          "bounds": s.bounds()[:2], "drift": s.stats()["norm_drift"]}
+            res[name] = {"hist": hist, "value": n_wp * steps / el, "terms": s.stats()["terms"],
+                         "bounds": s.bounds()[:2], "drift": s.stats()["norm_drift"]}
             s.close()
-    f, m = res[False], res[True]
+    f = res["fp64"]
     out["chebyshev_terms"] = f["terms"]
     out["fp64_wavepacket_steps_per_s"] = round(f["value"], 2)
-    out["mixed_wavepacket_steps_per_s"] = round(m["value"], 2)
-    out["pop_err_by_step"] = {str(check_every * (i + 1)): float(np.abs(pm - pf).max())
-                              for i, ((pf, _), (pm, _)) in enumerate(zip(f["hist"], m["hist"]))}
-    pf, ef = f["hist"][-1]
-    pm, em = m["hist"][-1]
-    out["final_pop_err"] = float(np.abs(pm - pf).max())
-    out["final_norm_err"] = float(np.abs(pm[:, -1] - pf[:, -1]).max())
-    out["final_energy_err_eV"] = float(np.abs(em - ef).max())
     out["fp64_norm_drift_last_step"] = f["drift"]
-    out["pop_err_bar_1e-6_met"] = out["final_pop_err"] <= 1e-6
+    for name, key in (("mixed", ""), ("f32_storage", "f32s_")):
+        m = res[name]
+        out[f"{name}_wavepacket_steps_per_s"] = round(m["value"], 2)
+        out[f"{key}pop_err_by_step"] = {str(check_every * (i + 1)): float(np.abs(pm - pf).max())
+                                        for i, ((pf, _), (pm, _)) in enumerate(zip(f["hist"], m["hist"]))}
+        pf, ef = f["hist"][-1]
+        pm, em = m["hist"][-1]
+        out[f"{key}final_pop_err"] = float(np.abs(pm - pf).max())
+        out[f"{key}final_norm_err"] = float(np.abs(pm[:, -1] - pf[:, -1]).max())
+        out[f"{key}final_energy_err_eV"] = float(np.abs(em - ef).max())
+        out[f"{key}pop_err_bar_1e-6_met"] = out[f"{key}final_pop_err"] <= 1e-6
     return out
 
 

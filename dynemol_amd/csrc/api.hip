@@ -88,8 +88,10 @@ struct dh_system {
     int next_graph = 0;
     bool use_graph = true;
     bool tuned = false;
-    int precision = 0;        // 0 = FP64, 1 = mixed (FP32 MFMA term GEMM)
-    float* H32 = nullptr;     // FP32 copy of H_orth for the mixed mode
+    // 0 = FP64; 1 = mixed (FP32 MFMA term GEMM); 2 = FP32-stored H_orth with
+    // FP64 products and sums
+    int precision = 0;
+    float* H32 = nullptr;     // FP32 copy of H_orth for modes 1 and 2
     bool h32_ready = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -586,7 +588,7 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
         if (trc) return trc;
     }
     s->h32_ready = false;
-    if (s->precision == 1) {
+    if (s->precision != 0) {
         launch_to_f32(s->Hth, s->H32, (long)ld * ld, st);
         s->h32_ready = true;
     }
@@ -775,9 +777,10 @@ static int series_launches(dh_system* s) {
     a.acc = s->acc;
     a.coef = s->coef;
     a.M32 = s->H32;
-    const bool mixed = s->precision == 1;
+    const int prec = s->precision;
     auto term = [&](const MatvecArgs& args) {
-        if (mixed) launch_gemm3(args, 1, reinterpret_cast<double*>(s->partial), st);
+        if (prec == 1) launch_gemm3(args, 1, reinterpret_cast<double*>(s->partial), st);
+        else if (prec == 2) launch_term_f32h(args, st);
         else launch_matvec(args, st);
     };
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
@@ -910,6 +913,7 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
     std::vector<double> n0(R), n1(R);
     // FP64 norm drift of one step that signals bounds not enclosing the
     // spectrum; the mixed mode's FP32 products drift ~1e-7 per step by themselves
+    // (mode 2 propagates with the exactly symmetric FP32-rounded H: FP64 drift)
     const double drift_tol = s->precision == 1 ? 1e-4 : 1e-9;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int rc = ensure_coefficients(s, dt);
@@ -1162,7 +1166,7 @@ extern "C" int dh_matvec_tuning(dh_system* s, int* out7) {
 }
 
 extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double* us_per_launch) {
-    CHECK_ARG(N > 0 && R >= 1 && R <= 128 && variant >= -1 && variant < matvec_variant_count() && reps > 0,
+    CHECK_ARG(N > 0 && R >= 1 && R <= 128 && variant >= -2 && variant < matvec_variant_count() && reps > 0,
               "dh_test_matvec_timing: bad argument");
     const int ld = round_ld(N);
     double* M = nullptr;
@@ -1192,13 +1196,14 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     DH_HIP(hipEventCreate(&e0));
     DH_HIP(hipEventCreate(&e1));
     float* M32 = nullptr;
-    if (variant == -1) {  // FP32 matrix-core GEMM (mixed precision)
+    if (variant < 0) {  // -1: FP32 matrix-core GEMM (mixed); -2: FP32-stored H, FP64 products
         DALLOC(M32, (size_t)ld * ld);
         DH_HIP(hipMemset(M32, 0, sizeof(float) * (size_t)ld * ld));
         a.M32 = M32;
     }
     auto run = [&]() {
         if (variant == -1) launch_gemm3(a, 1, reinterpret_cast<double*>(partial), nullptr);
+        else if (variant == -2) launch_term_f32h(a, nullptr);
         else launch_matvec_variant(a, variant, nullptr);
     };
     for (int w = 0; w < 3; ++w) run();
@@ -1216,11 +1221,12 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
 }
 
 extern "C" int dh_set_precision(dh_system* s, int mode) {
-    CHECK_ARG(s && (mode == 0 || mode == 1), "dh_set_precision: mode must be 0 (FP64) or 1 (mixed)");
+    CHECK_ARG(s && mode >= 0 && mode <= 2,
+              "dh_set_precision: mode must be 0 (FP64), 1 (mixed) or 2 (FP32-stored operator)");
     DH_HIP(hipSetDevice(s->ctx->device));
-    if (mode == 1 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
+    if (mode != 0 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
     s->precision = mode;
-    if (mode == 1 && s->ops_ready && !s->h32_ready) {
+    if (mode != 0 && s->ops_ready && !s->h32_ready) {
         launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);
         DH_HIP(hipStreamSynchronize(s->ctx->stream));
         s->h32_ready = true;

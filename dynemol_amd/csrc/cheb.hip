@@ -779,7 +779,9 @@ __device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, in
 constexpr int G3_T = 128;
 // k-stage depth: 16 (FP64) / 32 (FP32) keeps the double-buffered tiles at 74 KB of
 // LDS, so two workgroups share a CU and hide each other's barrier and load waits
-template <int PREC> struct G3 { static constexpr int BK = PREC == 0 ? 16 : 32; };
+// PREC: 0 = FP64, 1 = FP32 products and slice sums (mixed), 2 = FP32-stored H
+// widened to FP64 on load, FP64 products (the arithmetic of PREC 0)
+template <int PREC> struct G3 { static constexpr int BK = PREC == 1 ? 32 : 16; };
 // row pad: successive k-rows of an operand read start 32 banks apart (FP64) or
 // 16 banks apart (FP32), so the four rows a wave reads do not collide
 constexpr int G3_PAD = 16;
@@ -796,7 +798,7 @@ long gemm3_partial_doubles(int N, int ld, int R) { return (long)gemm3_splits(N, 
 
 template <int PREC>
 __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __restrict__ P, int KS) {
-    using T = typename std::conditional<PREC == 0, double, float>::type;
+    using T = typename std::conditional<PREC == 1, float, double>::type;
     constexpr int G3_BK = G3<PREC>::BK;
     __shared__ T As[2][G3_BK][G3_T + G3_PAD];
     __shared__ T Bs[2][G3_BK][G3_T + G3_PAD];
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
     const int wi = (w >> 1) * 64, wc = (w & 1) * 64;
     const double* X = reinterpret_cast<const double*>(a.x);
     const long xs = 2L * a.x_stride, xo = 2L * a.x_off;
-    using acc_t = typename std::conditional<PREC == 0, v4d, v4f>::type;
+    using acc_t = typename std::conditional<PREC == 1, v4f, v4d>::type;
     acc_t acc[4][4];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -826,7 +828,7 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
             T v = (T)0;
             if (gi < N && gk < khi) {
                 if constexpr (PREC == 0) v = a.M[gi + (long)a.ld * gk];
-                else v = a.M32[gi + (long)a.ld * gk];
+                else v = (T)a.M32[gi + (long)a.ld * gk];
             }
             ra[r] = v;
         }
@@ -867,7 +869,7 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
             for (int p = 0; p < 4; ++p)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if constexpr (PREC == 0)
+                    if constexpr (PREC != 1)
                         acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[q], av[p], acc[p][q], 0, 0, 0);
                     else
                         acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[q], av[p], acc[p][q], 0, 0, 0);
@@ -887,7 +889,7 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + p * 16 + (lane & 15);
-                const int gc = c0 + wc + q * 16 + (PREC == 0 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
+                const int gc = c0 + wc + q * 16 + (PREC != 1 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
                 if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = (double)acc[p][q][v];
             }
 }
@@ -909,12 +911,13 @@ __global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const d
 
 void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st) {
     const int S = gemm3_splits(a.N, a.R);
-    const int BK = prec == 0 ? G3<0>::BK : G3<1>::BK;
+    const int BK = prec == 1 ? G3<1>::BK : G3<0>::BK;
     const int KS = (((a.N + S - 1) / S) + BK - 1) / BK * BK;
     const int Sr = (a.N + KS - 1) / KS;
     dim3 grid((a.N + G3_T - 1) / G3_T, (2 * a.R + G3_T - 1) / G3_T, Sr);
     if (prec == 0) hipLaunchKernelGGL((gemm3_kernel<0>), grid, dim3(256), 0, st, a, P, KS);
-    else hipLaunchKernelGGL((gemm3_kernel<1>), grid, dim3(256), 0, st, a, P, KS);
+    else if (prec == 1) hipLaunchKernelGGL((gemm3_kernel<1>), grid, dim3(256), 0, st, a, P, KS);
+    else hipLaunchKernelGGL((gemm3_kernel<2>), grid, dim3(256), 0, st, a, P, KS);
     const long n = (long)a.N * a.R;
     hipLaunchKernelGGL(gemm3_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, P, Sr);
 }
@@ -1045,9 +1048,10 @@ bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2
 
 
 // x and H are read-only here and restrict-qualified, so wave-uniform x reads
-// become scalar loads.
-template <int R>
-__global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict__ Mg, const cplx* __restrict__ xg,
+// become scalar loads.  MT is H's storage type: double, or float for the
+// FP32-stored operator (widened on load; all arithmetic stays FP64).
+template <int R, typename MT>
+__global__ __launch_bounds__(256) void sym_tiles_kernel(const MT* __restrict__ Mg, const cplx* __restrict__ xg,
                                                         cplx* __restrict__ part, int ld, int N, int x_stride,
                                                         int x_off, int S) {
     constexpr int HP = TILE + 1;  // padded LDS column
@@ -1080,7 +1084,7 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
     auto fetch = [&](int J) {
         const int j0 = J * TILE + 16 * w;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) hn[s] = Mg[(long)(j0 + s) * ld + row];
+        for (int s = 0; s < 16; ++s) hn[s] = (double)Mg[(long)(j0 + s) * ld + row];
     };
     auto xload = [&](int j, double* xr, double* xi) {  // wave-uniform row j of x
         const cplx* p = xg + (long)min(j, N - 1) * x_stride + x_off;
@@ -1248,11 +1252,14 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecAr
     if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
 }
 
-template <int R>
+template <int R, typename MT = double>
 static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int nb = a.ld / TILE;
     const int S = sym_seg_for(nb);
-    hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
+    const MT* M;
+    if constexpr (std::is_same<MT, float>::value) M = a.M32;
+    else M = a.M;
+    hipLaunchKernelGGL((sym_tiles_kernel<R, MT>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, M, a.x,
                        a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
@@ -1369,6 +1376,22 @@ void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
 
 void launch_matvec(const MatvecArgs& a, hipStream_t st) {
     launch_matvec_variant(a, matvec_variant_for(a.N, a.R, a.tri), st);
+}
+
+// Term on the FP32-stored operator a.M32 with FP64 arithmetic: the symmetric
+// half-storage kernel for R <= 8, else the split-K GEMM with FP64 products.
+// One fixed kernel per shape, so results are reproducible run to run.
+void launch_term_f32h(const MatvecArgs& a, hipStream_t st) {
+    if (!sym_supported(a.R, a.tri)) {
+        launch_gemm3(a, 2, reinterpret_cast<double*>(a.partial), st);
+        return;
+    }
+    switch (a.R) {
+        case 1: launch_sym<1, float>(a, st); break;
+        case 2: launch_sym<2, float>(a, st); break;
+        case 4: launch_sym<4, float>(a, st); break;
+        default: launch_sym<8, float>(a, st); break;
+    }
 }
 
 // ------------------------------------------------------------------ Lanczos

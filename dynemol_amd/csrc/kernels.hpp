@@ -78,7 +78,7 @@ struct MatvecArgs {
     int k;                  // term index for EPI_CHEB
     int x_stride, x_off;    // column selection: uses x[i*x_stride + x_off + r], r < R
     int out_stride, out_off;
-    const float* M32;       // FP32 copy of M for the mixed-precision GEMM variant (may be null)
+    const float* M32;       // FP32 copy of M (mixed and FP32-storage precision modes; may be null)
 };
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
@@ -93,9 +93,11 @@ bool matvec_is_tuned(int N, int R, int tri);
 void matvec_force_variant(int v);
 int matvec_variant_count();
 void launch_cheb_gemm(const MatvecArgs& a, int prec, hipStream_t st);
-// split-K matrix-core term GEMM (prec 0: FP64, 1: FP32 products on M32); P holds
+// split-K matrix-core term GEMM (prec 0: FP64, 1: FP32 products on M32, 2: FP64
+// products on M32 widened to FP64); P holds
 // gemm3_partial_doubles(N, ld, R) doubles of partial products
 void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st);
+void launch_term_f32h(const MatvecArgs& a, hipStream_t st);
 long gemm3_partial_doubles(int N, int ld, int R);
 // Grouped column-block matvec over many systems (R in {1, 2, 4, 8, 16}); each
 // task is (descriptor index, column block of grouped_cols_per_block(R) columns).

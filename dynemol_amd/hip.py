@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DYNEMOL_HIP_LIB", os.path.join(HERE, "libdynemol_hip.so"))
 
 DH_MAT_S, DH_MAT_H, DH_MAT_L, DH_MAT_LINV, DH_MAT_HORTH, DH_MAT_HPRIME = range(6)
+DH_PREC_FP64, DH_PREC_MIXED, DH_PREC_F32_STORAGE = range(3)
 H_BAR = 6.58264e-4
 
 _lib = None
@@ -274,8 +275,11 @@ class System:
         keys = ["term", "energy", "L_phi_bra", "LinvT_phi", "LT_ket", "Linv_bra", "lanczos"]
         return dict(zip(keys, out.tolist()))
 
-    def set_precision(self, mixed: bool):
-        _check(lib().dh_set_precision(self.h, 1 if mixed else 0), "dh_set_precision")
+    def set_precision(self, mode):
+        """Term arithmetic: 0 / False = FP64, 1 / True = mixed (FP32 products),
+        2 = FP32-stored H_orth with FP64 products (DH_PREC_* constants)."""
+        m = int(mode)
+        _check(lib().dh_set_precision(self.h, m), "dh_set_precision")
 
     def set_timing(self, on=True):
         _check(lib().dh_set_timing(self.h, 1 if on else 0), "dh_set_timing")

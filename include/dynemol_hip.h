@@ -77,7 +77,10 @@ int dh_load_operators(dh_system *sys, const double *S, const double *h, int ld);
  * want_hprime also H' = S^-1 h (replaces ElHl_Chebyshev.f:205-210 and
  * GPU_Interface.cpp:880-949 + Taylor_gpu.cpp:687-696). */
 int dh_orthogonalize(dh_system *sys, int want_hprime);
-/* Copy a device matrix (DH_MAT_*) to host, N x N with leading dimension ld. */
+/* Copy a device matrix (DH_MAT_*) to host, N x N with leading dimension ld.
+ * DH_MAT_L and DH_MAT_LINV come back mirrored: the lower triangle holds the
+ * factor and the upper triangle its transpose (the storage the triangular
+ * products read), so take the lower triangle for L and L^-1. */
 int dh_get_matrix(dh_system *sys, int which, double *out, int ld);
 /* [E_min, E_max, theta_min, theta_max, residual_min, residual_max] (eV). */
 int dh_get_bounds(dh_system *sys, double *out6);
@@ -87,7 +90,10 @@ int dh_set_lanczos(dh_system *sys, int steps, double rel_margin);
 /* Arithmetic of the Chebyshev term products: 0 = FP64 (default); 1 = mixed:
  * FP32 matrix-core products with FP32 accumulation on an FP32 copy of H_orth,
  * while the recurrence, Bessel coefficients, accumulators, norms and
- * observables stay FP64.  No counterpart in the reference (all FP64). */
+ * observables stay FP64; 2 = FP32-stored operator: the terms read an FP32
+ * copy of H_orth (half the bytes of the HBM-bound few-wavepacket term) and
+ * widen it to FP64, so products and sums are FP64 and the only error is the
+ * rounding of H_orth itself.  No counterpart in the reference (all FP64). */
 int dh_set_precision(dh_system *sys, int mode);
 
 /* Wavepacket wp's AO-basis Psi_t_bra and Psi_t_ket, N complex each. */

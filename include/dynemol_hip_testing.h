@@ -23,7 +23,8 @@ int dh_test_copy(long long bytes, int repeats, double *ms_per_copy);
 int dh_test_read(long long bytes, int repeats, double *ms_per_read);
 /* Average launch time (microseconds, HIP events) of one matvec variant on an
  * N x N FP64 operator with R complex right-hand sides (Chebyshev epilogue);
- * variant -1 times the FP32 matrix-core GEMM of the mixed mode. */
+ * variant -1 times the FP32 matrix-core GEMM of the mixed mode, -2 the term
+ * kernel of the FP32-stored operator mode. */
 int dh_test_matvec_timing(int N, int R, int variant, int reps, double *us_per_launch);
 /* Force every matvec to one kernel variant (-1 restores the autotuned choice). */
 int dh_test_force_matvec_variant(int variant);

@@ -154,9 +154,10 @@ def test_matvec_variants_propagate_exactly(variant, n_wp):
         hip.force_matvec_variant(-1)
 
 
-@pytest.mark.parametrize("n,n_wp", [(1100, 1), (1100, 2), (1100, 4)])
+@pytest.mark.parametrize("n,n_wp", [(1100, 1), (1100, 2), (1100, 4), (1200, 8)])
 def test_symmetric_half_storage_variant_large(n, n_wp):
-    """Variant 16 (lower-triangle tiles, two-phase) over many block rows and segments."""
+    """Variant 16 (lower-triangle tiles, two-phase) over many block rows and
+    segments; even and odd block counts (18 and 19 blocks of 64)."""
     rng = np.random.default_rng(n + n_wp)
     _say(f"symmetric variant N={n} n_wp={n_wp}")
     X = rng.normal(size=(n, n))

@@ -87,3 +87,41 @@ def test_mixed_precision_tracks_fp64(c1):
         assert np.abs(k32 - k64).max() < 1e-4 * np.abs(k64).max()
         assert np.abs(p32 - p64).max() < 1e-4
         assert np.abs(p32 - p64).max() > 0.0  # the mixed path really ran in FP32
+
+
+@pytest.mark.parametrize("n_wp", [2, 20])
+def test_fp32_storage_is_exact_for_rounded_operator(c1, n_wp):
+    """Mode 2 reads H_orth rounded to FP32 and does FP64 arithmetic: it must be
+    the exact propagator of that rounded operator to FP64 accuracy.  n_wp = 2
+    (4 RHS) runs the symmetric half-storage kernel, n_wp = 20 (40 RHS) the
+    split-K GEMM with widened FP32 operands."""
+    st, b, S, h, E, C = c1
+    nsteps = 3
+    _say(f"FP32-stored operator, {n_wp} wavepackets, {nsteps} steps")
+    kets = _states(C, S, n_wp, 3)
+    with hip.Context(0) as ctx:
+        s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
+        s.set_precision(hip.DH_PREC_F32_STORAGE)
+        s.build(st.coords)
+        L = np.tril(s.matrix(hip.DH_MAT_L))  # stored mirrored
+        Hq = s.matrix(hip.DH_MAT_HORTH).astype(np.float32).astype(np.float64)
+        for w, k in enumerate(kets):
+            s.set_wavepacket(w, S @ k, k)
+        t = 0.0
+        for _ in range(nsteps):
+            s.propagate(t, t + DT)
+            t += DT
+        out = [s.wavepacket(w) for w in range(n_wp)]
+        s.close()
+    assert np.abs(L @ L.T - S).max() < 1e-12
+    lam, V = np.linalg.eigh(Hq)
+    U = V @ (np.exp(-1j * lam * t / O.H_BAR)[:, None] * V.T)  # exp(-i Hq t / hbar)
+    Uex = C @ (np.exp(-1j * E * t / O.H_BAR)[:, None] * (C.T @ S))  # unrounded operator, AO basis
+    for w, k in enumerate(kets):
+        bra, ket = out[w]
+        ke = sl.solve_triangular(L, U @ (L.T @ k), trans="T", lower=True)  # L^-T U L^T k
+        be = L @ (U @ sl.solve_triangular(L, S @ k, lower=True))  # L U L^-1 S k
+        assert np.abs(ket - ke).max() <= 1e-10 * np.abs(ke).max()
+        assert np.abs(bra - be).max() <= 1e-10 * np.abs(be).max()
+        # and it is not the FP64 operator: rounding H_orth moves the result
+        assert np.abs(ket - Uex @ k).max() > 1e-12 * np.abs(ke).max()
