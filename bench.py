@@ -30,7 +30,10 @@ HBM_PEAK_GBS = 8000.0
 # 4 RHS) from rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the same
 # counter reads for the library's plain copy kernel over 256 MiB (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000): profiles/cheb_term_sym_tio2_N4096_counters.md.
-MEASURED_TRAFFIC_BYTES = {16: int(((35409.8125 + 5487.5) * 2.0 + (10924.0 + 256.0)) * 1024)}
+# Variant 17 likewise: profiles/cheb_term_symm_tio2_N4096_counters.md.
+MEASURED_TRAFFIC_BYTES = {16: int(((35409.8125 + 5487.5) * 2.0 + (10924.0 + 256.0)) * 1024),
+                          17: int(((34796.36781939338 + 5874.9525547445255) * 2.0
+                                   + (10924.001321231617 + 508.26277372262774)) * 1024)}
 
 
 def log(msg):
@@ -113,7 +116,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     }
 
 
-SYM_VARIANT = 16
+SYM_VARIANTS = (16, 17)  # symmetric half-storage term kernels (vector FMAs, matrix-core tiles)
 
 
 def term_bytes(N, R, variant):
@@ -125,7 +128,7 @@ def term_bytes(N, R, variant):
     diagonal, nb (nb + 1) / 2 of them.
     """
     vec = 80 * N * R
-    if variant == SYM_VARIANT:
+    if variant in SYM_VARIANTS:
         nb = (N + 63) // 64
         return 8 * 64 * 64 * nb * (nb + 1) // 2 + vec
     return 8 * N * N + vec
@@ -453,7 +456,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_BYTES.get(variant) if N == 4096 else None,
                      "kernel": f"Chebyshev term matvec, FP64, 4 RHS (autotuned variant {variant}: "
-                               f"{'lower-triangle tiles of symmetric H' if variant == SYM_VARIANT else 'dense H'})",
+                               f"{'lower-triangle tiles of symmetric H' if variant in SYM_VARIANTS else 'dense H'})",
                      "bytes_per_launch": bytes_term, "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "dense_equivalent_GBs": round((8 * N * N + 80 * N * R) / avg_launch_s / 1e9, 1)},
         "propagation_only_value": round(prop_only, 3),

@@ -1032,16 +1032,19 @@ __host__ __device__ inline int sym_segment_start(int B, int S) {
 
 static int sym_segment_count(int nb, int S) { return sym_segment_start(nb, S); }
 
-// shortest segment whose grid fits four resident workgroups per CU
-static int sym_seg_for(int nb) {
+// shortest segment whose grid fits per_cu resident workgroups per CU (256 CUs):
+// four for the vector kernel, three for the matrix-core kernel (its LDS)
+static int sym_seg_for(int nb, int per_cu = 4) {
     int S = 1;
-    while (sym_segment_count(nb, S) > 4 * 256) ++S;
+    while (sym_segment_count(nb, S) > per_cu * 256) ++S;
     return S;
 }
 
+// the partial buffer of both symmetric kernels: the segment count only falls
+// as per_cu falls, so the four-per-CU grid is the largest
 long sym_partial_cplx(int ld, int R) {
     const long nb = ld / TILE;
-    return (nb * (nb - 1) / 2 + sym_segment_count((int)nb, sym_seg_for((int)nb))) * TILE * R;
+    return (nb * (nb - 1) / 2 + sym_segment_count((int)nb, sym_seg_for((int)nb, 4))) * TILE * R;
 }
 
 bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2 || R == 4 || R == 8); }
@@ -1264,9 +1267,130 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
 
+// ------------------------------------------------------------------ symmetric, matrix-core tiles
+// Variant 17: the segments and partial layout of the symmetric kernel above
+// (so sym_reduce_kernel is its phase 2), with both tile products on FP64 MFMA
+// 16x16x4.  Wave w of the workgroup owns rows 16w..16w+15 of the row product
+// and columns 16w..16w+15 of the column product, each complete over the tile:
+//  * rows:    Y^T[rhs][i] += sum_k X_J^T[rhs][k] H[i][k]; the B operand
+//             (lane l: H[16w + (l & 15)][4c + (l >> 4)]) is the wave's own
+//             coalesced load (4 columns x 128 B per instruction);
+//  * columns: Z^T[rhs][j] += sum_k X_I^T[rhs][k] H[k][j]; the B operand
+//             (lane l: H[4r + (l >> 4)][16w + (l & 15)]) comes from the tile in
+//             LDS (column-major, 65-double columns: conflict-free both ways).
+// x_J (per tile) and x_I (per segment) are staged in LDS as 64 x 2R doubles
+// and read as the A operands (lane l: row 4c + (l >> 4), real column l & 15,
+// clamped to 2R - 1; MFMA output rows >= 2R are discarded).  No scalar loads
+// and no cross-wave reductions; partial sums are written straight from the
+// accumulators (lane l, value v: real column (l >> 4) + 4v, row/column l & 15).
+template <int R>
+__global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
+                                                         double* __restrict__ part, int ld, int N, int x_stride,
+                                                         int x_off, int S) {
+    constexpr int HP = TILE + 1;
+    constexpr int C2 = 2 * R;                   // real right-hand-side columns
+    constexpr int XE = TILE * C2;               // doubles of one staged x block
+    constexpr int XPT = (XE + 255) / 256;       // staged x doubles per thread
+    __shared__ double Hs[TILE * HP];
+    __shared__ double xs[XE];                   // x_J of the current tile
+    __shared__ double xis[XE];                  // x_I of the segment
+    const int nb = ld / TILE;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sym_segment_start(mid, S) <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const int I = lo;
+    const int Ja = ((int)blockIdx.x - sym_segment_start(I, S)) * S, Jb = min(I + 1, Ja + S);
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i0 = I * TILE;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int xc = min(lr, C2 - 1);             // A-operand real column of this lane
+    const long xs2 = 2L * x_stride, xo2 = 2L * x_off;
+    // H's padding is [0 0; 0 I] and x rows >= N are read as row N - 1 (finite
+    // data): rows >= N of x never reach rows < N of the result.
+    auto xfetch = [&](int r0, double* v) {      // 64 x 2R block of x from row r0
+#pragma unroll
+        for (int u = 0; u < XPT; ++u) {
+            const int e = t + 256 * u;
+            if (XE % 256 == 0 || e < XE) v[u] = xg[(long)min(r0 + e / C2, N - 1) * xs2 + xo2 + e % C2];
+        }
+    };
+    double hn[16], xn[XPT];
+    auto fetch = [&](int J) {
+        const int j0 = J * TILE;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) hn[c] = Mg[(long)(j0 + 4 * c + lk) * ld + i0 + 16 * w + lr];
+        xfetch(j0, xn);
+    };
+    {
+        double xi[XPT];
+        xfetch(i0, xi);
+#pragma unroll
+        for (int u = 0; u < XPT; ++u) {
+            const int e = t + 256 * u;
+            if (XE % 256 == 0 || e < XE) xis[e] = xi[u];
+        }
+    }
+    v4d yacc = {0.0, 0.0, 0.0, 0.0};
+    fetch(Ja);
+    for (int J = Ja; J < Jb; ++J) {
+        double hc[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) hc[c] = hn[c];
+        __syncthreads();  // previous tile's LDS readers are done
+#pragma unroll
+        for (int c = 0; c < 16; ++c) Hs[(4 * c + lk) * HP + 16 * w + lr] = hc[c];
+#pragma unroll
+        for (int u = 0; u < XPT; ++u) {
+            const int e = t + 256 * u;
+            if (XE % 256 == 0 || e < XE) xs[e] = xn[u];
+        }
+        if (J + 1 < Jb) fetch(J + 1);
+        __syncthreads();  // tile and x_J staged
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+            yacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[(4 * c + lk) * C2 + xc], hc[c], yacc, 0, 0, 0);
+        if (J == I) continue;  // diagonal tile: the row sums cover it
+        v4d zacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            zacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xis[(4 * r + lk) * C2 + xc], Hs[(16 * w + lr) * HP + 4 * r + lk],
+                                                        zacc, 0, 0, 0);
+        double* pz = part + 2 * (((long)I * (I - 1) / 2 + J) * TILE * R + (long)(16 * w + lr) * R);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = lk + 4 * v;
+            if (m < C2) pz[m] = zacc[v];
+        }
+    }
+    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + lr) * R);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int m = lk + 4 * v;
+        if (m < C2) py[m] = yacc[v];
+    }
+}
+
+template <int R>
+static void launch_symm(const MatvecArgs& a, hipStream_t st) {
+    const int nb = a.ld / TILE;
+    const int S = sym_seg_for(nb, 3);
+    hipLaunchKernelGGL((symm_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
+                       reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
+                       a.x_stride, a.x_off, S);
+    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
+}
+
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
-// counters); 1..4 = column-block kernels of different shapes.
-constexpr int MV_VARIANTS = 17;
+// counters); 1..4 and 9..14 = column-block kernels of different shapes and LDS
+// schedules; 5..8 = two-phase split-K; 15 = split-K matrix-core GEMM (full
+// operator); 16 = symmetric half storage, vector FMAs; 17 = symmetric half
+// storage, matrix-core tile products.
+constexpr int MV_VARIANTS = 18;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -1329,6 +1453,20 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
+    if (variant == 17) {
+        // symmetric kernel with matrix-core tile products; other shapes use variant 1
+        if (!sym_supported(a.R, a.tri)) {
+            launch_matvec_variant(a, 1, st);
+            return;
+        }
+        switch (a.R) {
+            case 1: launch_symm<1>(a, st); break;
+            case 2: launch_symm<2>(a, st); break;
+            case 4: launch_symm<4>(a, st); break;
+            default: launch_symm<8>(a, st); break;
+        }
+        return;
+    }
     if (variant == 16) {
         // symmetric half-storage two-phase kernel; other shapes use the column-block kernel
         if (!sym_supported(a.R, a.tri)) {

@@ -119,7 +119,7 @@ def test_cholesky_reports_non_spd():
     assert info == 71
 
 
-@pytest.mark.parametrize("variant", list(range(17)))
+@pytest.mark.parametrize("variant", list(range(18)))
 @pytest.mark.parametrize("n_wp", [1, 2, 3, 4])
 def test_matvec_variants_propagate_exactly(variant, n_wp):
     """Every matvec kernel variant the autotuner may pick gives the exact propagator."""
@@ -154,12 +154,14 @@ def test_matvec_variants_propagate_exactly(variant, n_wp):
         hip.force_matvec_variant(-1)
 
 
+@pytest.mark.parametrize("variant", [16, 17])
 @pytest.mark.parametrize("n,n_wp", [(1100, 1), (1100, 2), (1100, 4), (1200, 8)])
-def test_symmetric_half_storage_variant_large(n, n_wp):
-    """Variant 16 (lower-triangle tiles, two-phase) over many block rows and
-    segments; even and odd block counts (18 and 19 blocks of 64)."""
+def test_symmetric_half_storage_variant_large(n, n_wp, variant):
+    """Variants 16 (vector FMAs) and 17 (matrix-core tile products): lower-triangle
+    tiles, two-phase, over many block rows and segments; even and odd block
+    counts (18 and 19 blocks of 64)."""
     rng = np.random.default_rng(n + n_wp)
-    _say(f"symmetric variant N={n} n_wp={n_wp}")
+    _say(f"symmetric variant {variant} N={n} n_wp={n_wp}")
     X = rng.normal(size=(n, n))
     S = X @ X.T / n + np.eye(n)
     A = rng.normal(size=(n, n))
@@ -167,7 +169,7 @@ def test_symmetric_half_storage_variant_large(n, n_wp):
     import scipy.linalg as sl
     E, C = sl.eigh(h, S)
     dt = 5e-4
-    hip.force_matvec_variant(16)
+    hip.force_matvec_variant(variant)
     try:
         ctx = hip.Context(0)
         s = hip.System(ctx, None, n_wp=n_wp, n_frag=1, n_orb=n)
