@@ -1283,7 +1283,16 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
 // clamped to 2R - 1; MFMA output rows >= 2R are discarded).  No scalar loads
 // and no cross-wave reductions; partial sums are written straight from the
 // accumulators (lane l, value v: real column (l >> 4) + 4v, row/column l & 15).
-template <int R>
+//
+// MF = 1 (variant 18) runs the same products as FP64 MFMA 4x4x4 in 4 blocks,
+// whose lanes are A: 16k + 4b + i, B: 16k + 4b + j, D: 16i + 4b + j (checked
+// on the device).  Block b covers rows (or columns) 4b..4b+3 of the wave's 16,
+// so the A operand of the row product is again the wave's own load
+// (4b + i = l & 15) and that of the column product the same LDS read; B is x
+// in groups of 4 real columns (lane l: column 4h + (l & 3)).  For R <= 4 this
+// does no discarded work: ceil(2R / 4) products per k-step of 16 cycles each,
+// against one 64-cycle 16x16x4 whose rows >= 2R are discarded.
+template <int R, int MF>
 __global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
                                                          double* __restrict__ part, int ld, int N, int x_stride,
                                                          int x_off, int S) {
@@ -1334,7 +1343,12 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restric
             if (XE % 256 == 0 || e < XE) xis[e] = xi[u];
         }
     }
+    constexpr int G = (C2 + 3) / 4;             // MF = 1: groups of 4 real columns
     v4d yacc = {0.0, 0.0, 0.0, 0.0};
+    double yacc4[G];
+#pragma unroll
+    for (int h = 0; h < G; ++h) yacc4[h] = 0.0;
+    const int xq = lane & 3;
     fetch(Ja);
     for (int J = Ja; J < Jb; ++J) {
         double hc[16];
@@ -1350,36 +1364,72 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restric
         }
         if (J + 1 < Jb) fetch(J + 1);
         __syncthreads();  // tile and x_J staged
+        if constexpr (MF == 0) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-            yacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[(4 * c + lk) * C2 + xc], hc[c], yacc, 0, 0, 0);
+            for (int c = 0; c < 16; ++c)
+                yacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[(4 * c + lk) * C2 + xc], hc[c], yacc, 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+#pragma unroll
+                for (int h = 0; h < G; ++h)
+                    yacc4[h] = __builtin_amdgcn_mfma_f64_4x4x4f64(hc[c], xs[(4 * c + lk) * C2 + min(4 * h + xq, C2 - 1)],
+                                                                  yacc4[h], 0, 0, 0);
+        }
         if (J == I) continue;  // diagonal tile: the row sums cover it
-        v4d zacc = {0.0, 0.0, 0.0, 0.0};
+        const long tI = (long)I * (I - 1) / 2 + J;
+        if constexpr (MF == 0) {
+            v4d zacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-            zacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xis[(4 * r + lk) * C2 + xc], Hs[(16 * w + lr) * HP + 4 * r + lk],
-                                                        zacc, 0, 0, 0);
-        double* pz = part + 2 * (((long)I * (I - 1) / 2 + J) * TILE * R + (long)(16 * w + lr) * R);
+            for (int r = 0; r < 16; ++r)
+                zacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xis[(4 * r + lk) * C2 + xc],
+                                                            Hs[(16 * w + lr) * HP + 4 * r + lk], zacc, 0, 0, 0);
+            double* pz = part + 2 * (tI * TILE * R + (long)(16 * w + lr) * R);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = lk + 4 * v;
-            if (m < C2) pz[m] = zacc[v];
+            for (int v = 0; v < 4; ++v) {
+                const int m = lk + 4 * v;
+                if (m < C2) pz[m] = zacc[v];
+            }
+        } else {
+            double zacc4[G];
+#pragma unroll
+            for (int h = 0; h < G; ++h) zacc4[h] = 0.0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const double hv = Hs[(16 * w + lr) * HP + 4 * r + lk];
+#pragma unroll
+                for (int h = 0; h < G; ++h)
+                    zacc4[h] = __builtin_amdgcn_mfma_f64_4x4x4f64(hv, xis[(4 * r + lk) * C2 + min(4 * h + xq, C2 - 1)],
+                                                                  zacc4[h], 0, 0, 0);
+            }
+            // D lane l: column 16w + 4 ((l >> 2) & 3) + (l >> 4), real column 4h + (l & 3)
+            double* pz = part + 2 * (tI * TILE * R + (long)(16 * w + 4 * ((lane >> 2) & 3) + lk) * R);
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                if (4 * h + xq < C2) pz[4 * h + xq] = zacc4[h];
         }
     }
     const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
-    double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + lr) * R);
+    if constexpr (MF == 0) {
+        double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + lr) * R);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int m = lk + 4 * v;
-        if (m < C2) py[m] = yacc[v];
+        for (int v = 0; v < 4; ++v) {
+            const int m = lk + 4 * v;
+            if (m < C2) py[m] = yacc[v];
+        }
+    } else {
+        double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + 4 * ((lane >> 2) & 3) + lk) * R);
+#pragma unroll
+        for (int h = 0; h < G; ++h)
+            if (4 * h + xq < C2) py[4 * h + xq] = yacc4[h];
     }
 }
 
-template <int R>
+template <int R, int MF>
 static void launch_symm(const MatvecArgs& a, hipStream_t st) {
     const int nb = a.ld / TILE;
     const int S = sym_seg_for(nb, 3);
-    hipLaunchKernelGGL((symm_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
+    hipLaunchKernelGGL((symm_tiles_kernel<R, MF>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off, S);
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
@@ -1389,8 +1439,8 @@ static void launch_symm(const MatvecArgs& a, hipStream_t st) {
 // counters); 1..4 and 9..14 = column-block kernels of different shapes and LDS
 // schedules; 5..8 = two-phase split-K; 15 = split-K matrix-core GEMM (full
 // operator); 16 = symmetric half storage, vector FMAs; 17 = symmetric half
-// storage, matrix-core tile products.
-constexpr int MV_VARIANTS = 18;
+// storage, matrix-core tile products (16x16x4); 18 = the same with 4x4x4 products.
+constexpr int MV_VARIANTS = 19;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -1453,17 +1503,19 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
-    if (variant == 17) {
-        // symmetric kernel with matrix-core tile products; other shapes use variant 1
+    if (variant == 17 || variant == 18) {
+        // symmetric kernel with matrix-core tile products (16x16x4 or 4x4x4);
+        // other shapes use variant 1
         if (!sym_supported(a.R, a.tri)) {
             launch_matvec_variant(a, 1, st);
             return;
         }
+        const bool small = variant == 18;
         switch (a.R) {
-            case 1: launch_symm<1>(a, st); break;
-            case 2: launch_symm<2>(a, st); break;
-            case 4: launch_symm<4>(a, st); break;
-            default: launch_symm<8>(a, st); break;
+            case 1: small ? launch_symm<1, 1>(a, st) : launch_symm<1, 0>(a, st); break;
+            case 2: small ? launch_symm<2, 1>(a, st) : launch_symm<2, 0>(a, st); break;
+            case 4: small ? launch_symm<4, 1>(a, st) : launch_symm<4, 0>(a, st); break;
+            default: small ? launch_symm<8, 1>(a, st) : launch_symm<8, 0>(a, st); break;
         }
         return;
     }
