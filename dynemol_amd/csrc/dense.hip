@@ -233,73 +233,136 @@ void launch_gemm(const Gemm& g, hipStream_t st) {
 }
 
 // Factor the 64 x 64 diagonal block at (j0, j0) in place (lower) and write its
-// inverse to Dinv[blk].  1024 threads; thread t owns the four elements
-// (r, c) = (t % 64, t / 64 + 16 q) of the block, held column-major in LDS.
-//   factor : unscaled right-looking elimination, S_rc -= S_rj S_cj / S_jj for
-//            c > j, r >= c (step j reads column j, writes later columns: one
-//            barrier per step), then L_rc = S_rc / sqrt(S_cc);
-//   inverse: L X = I by right-looking forward substitution over rows,
-//            B_rc -= L_ri B_ic / L_ii for r > i, then X_rc = B_rc / L_rr.
+// inverse to Dinv[blk].  1024 threads, the block in LDS (column-major, padded),
+// blocked by 16 so that only 20 barriers separate the serial parts:
+//   factor, for each 16-column sub-panel k0:
+//     1. wave 0 factors the 16 x 16 diagonal sub-block, lane r holding row r in
+//        registers and taking column entries of other rows by shuffles;
+//     2. one thread per row below solves L_i,panel = A_i,panel L_D^-T by forward
+//        substitution over the 16 columns;
+//     3. all threads apply the rank-16 update to the lower part below;
+//   inverse X = L^-1:
+//     4. waves 0..3 invert the four 16 x 16 diagonal sub-blocks (lane c solves
+//        column c by forward substitution);
+//     5. sub-diagonal d = 1, 2, 3: T_IJ = sum_{K=J..I-1} L_IK X_KJ, then
+//        X_IJ = -X_II T_IJ, for all sub-blocks I - J = d at once.
+// The first non-positive pivot is reported as in LAPACK's potrf (1-based).
+constexpr int PD_P = TILE + 1;  // padded LDS column
+constexpr int PD_B = 16;        // sub-block size
 __global__ __launch_bounds__(1024) void potrf_diag_kernel(double* A, int ld, int j0, double* Dinv, int* info) {
-    __shared__ double Ls[TILE * TILE];  // Ls[c * 64 + r]
-    __shared__ double Bs[TILE * TILE];
+    __shared__ double Ls[TILE * PD_P];  // Ls[c * PD_P + r] = block (r, c)
+    __shared__ double Xs[TILE * PD_P];  // inverse, same layout
+    __shared__ double Ts[TILE * PD_P];  // products of step 5
     __shared__ int bad;
-    const int t = threadIdx.x, r = t & 63, cq = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) bad = TILE + 1;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = cq + 16 * q;
-        Ls[c * TILE + r] = A[(long)(j0 + r) + (long)ld * (j0 + c)];
-        Bs[c * TILE + r] = r == c ? 1.0 : 0.0;
+    for (int e = t; e < TILE * TILE; e += 1024) {
+        const int r = e % TILE, c = e / TILE;
+        Ls[c * PD_P + r] = A[(long)(j0 + r) + (long)ld * (j0 + c)];
+        Xs[c * PD_P + r] = 0.0;
     }
     __syncthreads();
-    for (int j = 0; j < TILE - 1; ++j) {
-        const double d = Ls[j * TILE + j];
-        const double s = d > 0.0 ? Ls[j * TILE + r] / d : 0.0;
+    for (int k0 = 0; k0 < TILE; k0 += PD_B) {
+        if (w == 0) {  // 1. diagonal sub-block (wave-uniform branch: no barrier inside)
+            const int r = lane < PD_B ? lane : 0;
+            double a[PD_B];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = cq + 16 * q;
-            if (c > j && r >= c) Ls[c * TILE + r] -= s * Ls[j * TILE + c];
+            for (int c = 0; c < PD_B; ++c) a[c] = Ls[(k0 + c) * PD_P + k0 + r];
+#pragma unroll
+            for (int j = 0; j < PD_B; ++j) {
+                const double d = __shfl(a[j], j);
+                const bool ok = d > 0.0;
+                if (!ok && lane == 0) atomicMin(&bad, k0 + j + 1);
+                const double sd = ok ? sqrt(d) : 1.0;
+                a[j] = lane == j ? sd : (lane > j ? a[j] / sd : a[j]);
+#pragma unroll
+                for (int c = j + 1; c < PD_B; ++c) {
+                    const double lcj = __shfl(a[j], c);  // L_cj from row c
+                    if (lane >= c) a[c] -= a[j] * lcj;
+                }
+            }
+            if (lane < PD_B) {
+#pragma unroll
+                for (int c = 0; c < PD_B; ++c) Ls[(k0 + c) * PD_P + k0 + lane] = c <= lane ? a[c] : 0.0;
+            }
+        }
+        __syncthreads();
+        const int n2 = TILE - k0 - PD_B;  // rows below the sub-block
+        if (n2 <= 0) break;
+        if (t < n2) {  // 2. sub-panel solve, one row per thread
+            const int i = k0 + PD_B + t;
+            double x[PD_B];
+#pragma unroll
+            for (int c = 0; c < PD_B; ++c) {
+                double v = Ls[(k0 + c) * PD_P + i];
+#pragma unroll
+                for (int k = 0; k < c; ++k) v -= x[k] * Ls[(k0 + k) * PD_P + k0 + c];
+                x[c] = v / Ls[(k0 + c) * PD_P + k0 + c];
+            }
+#pragma unroll
+            for (int c = 0; c < PD_B; ++c) Ls[(k0 + c) * PD_P + i] = x[c];
+        }
+        __syncthreads();
+        for (int e = t; e < n2 * n2; e += 1024) {  // 3. rank-16 update, lower part
+            const int i = k0 + PD_B + e % n2, j = k0 + PD_B + e / n2;
+            if (i < j) continue;
+            double v = Ls[j * PD_P + i];
+#pragma unroll
+            for (int k = 0; k < PD_B; ++k) v -= Ls[(k0 + k) * PD_P + i] * Ls[(k0 + k) * PD_P + j];
+            Ls[j * PD_P + i] = v;
         }
         __syncthreads();
     }
-    double lv[4];
+    if (w < TILE / PD_B && lane < PD_B) {  // 4. diagonal sub-block inverses
+        const int b0 = PD_B * w, c = lane;
+        double x[PD_B];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = cq + 16 * q;
-        const double d = Ls[c * TILE + c];
-        if (!(d > 0.0) && r == 0) atomicMin(&bad, c + 1);  // first non-positive pivot
-        const double sd = d > 0.0 ? sqrt(d) : 1.0;
-        lv[q] = r > c ? Ls[c * TILE + r] / sd : (r == c ? sd : 0.0);
+        for (int i = 0; i < PD_B; ++i) {
+            double v = i == c ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) v -= Ls[(b0 + k) * PD_P + b0 + i] * x[k];
+            x[i] = v / Ls[(b0 + i) * PD_P + b0 + i];
+        }
+#pragma unroll
+        for (int i = 0; i < PD_B; ++i) Xs[(b0 + c) * PD_P + b0 + i] = x[i];
     }
     __syncthreads();
+    constexpr int NBK = TILE / PD_B;
+    for (int d = 1; d < NBK; ++d) {  // 5. off-diagonal sub-blocks by sub-diagonal
+        const int nblk = NBK - d;
+        for (int e = t; e < nblk * PD_B * PD_B; e += 1024) {
+            const int J = e / (PD_B * PD_B), I = J + d, i = e % PD_B, j = (e / PD_B) % PD_B;
+            double v = 0.0;
+            for (int k = PD_B * J; k < PD_B * I; ++k) v += Ls[k * PD_P + PD_B * I + i] * Xs[(PD_B * J + j) * PD_P + k];
+            Ts[(PD_B * J + j) * PD_P + PD_B * I + i] = v;
+        }
+        __syncthreads();
+        for (int e = t; e < nblk * PD_B * PD_B; e += 1024) {
+            const int J = e / (PD_B * PD_B), I = J + d, i = e % PD_B, j = (e / PD_B) % PD_B;
+            double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Ls[(cq + 16 * q) * TILE + r] = lv[q];
-    __syncthreads();
-    for (int i = 0; i < TILE - 1; ++i) {
-        const double f = r > i ? Ls[i * TILE + r] / Ls[i * TILE + i] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = cq + 16 * q;
-            if (r > i && c <= i) Bs[c * TILE + r] -= f * Bs[c * TILE + i];
+            for (int k = 0; k < PD_B; ++k)
+                v -= Xs[(PD_B * I + k) * PD_P + PD_B * I + i] * Ts[(PD_B * J + j) * PD_P + PD_B * I + k];
+            Xs[(PD_B * J + j) * PD_P + PD_B * I + i] = v;
         }
         __syncthreads();
     }
     const int blk = j0 / TILE;
-    const double lrr = Ls[r * TILE + r];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = cq + 16 * q;
-        A[(long)(j0 + r) + (long)ld * (j0 + c)] = lv[q];
-        Dinv[(long)blk * TILE * TILE + r + (long)TILE * c] = r >= c ? Bs[c * TILE + r] / lrr : 0.0;
+    for (int e = t; e < TILE * TILE; e += 1024) {
+        const int r = e % TILE, c = e / TILE;
+        A[(long)(j0 + r) + (long)ld * (j0 + c)] = r >= c ? Ls[c * PD_P + r] : 0.0;
+        Dinv[(long)blk * TILE * TILE + r + (long)TILE * c] = r >= c ? Xs[c * PD_P + r] : 0.0;
     }
     if (t == 0 && bad <= TILE && *info == 0) *info = j0 + bad;
 }
 
-// zero the strictly upper part of the panel columns j0..j0+63 above the diagonal block
-__global__ void zero_upper_panel_kernel(double* A, int ld, int j0) {
-    const int c = blockIdx.x;  // column within the panel
-    for (int i = threadIdx.x; i < j0; i += blockDim.x) A[(long)i + (long)ld * (j0 + c)] = 0.0;
+// zero the blocks above the diagonal blocks (column j: rows 0 .. 64 floor(j / 64) - 1),
+// once the factorization is done: the trailing updates run on lower tiles of
+// 128 and leave values in some of these blocks, and nothing reads them before
+__global__ void zero_upper_blocks_kernel(double* A, int ld) {
+    const int j = blockIdx.x;
+    const int top = (j / TILE) * TILE;
+    for (int i = threadIdx.x; i < top; i += blockDim.x) A[(long)i + (long)ld * j] = 0.0;
 }
 
 void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st) {
@@ -307,7 +370,6 @@ void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st)
     for (int b = 0; b < nb; ++b) {
         const int j0 = b * TILE;
         hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, ld, j0, Dinv, info);
-        if (j0 > 0) hipLaunchKernelGGL(zero_upper_panel_kernel, dim3(TILE), dim3(256), 0, st, A, ld, j0);
         const int rest = ld - j0 - TILE;
         if (rest <= 0) continue;
         // panel: A21 <- A21 * L11^-T (Dinv holds L11^-1).  In place is safe:
@@ -329,6 +391,7 @@ void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st)
         u.alpha = -1.0; u.beta = 1.0; u.lower_only = 1; u.batch = 1;
         launch_gemm(u, st);
     }
+    if (nb > 1) hipLaunchKernelGGL(zero_upper_blocks_kernel, dim3(ld), dim3(256), 0, st, A, ld);
 }
 
 // L^-1 by recursive doubling over the 64 x 64 diagonal blocks (whose inverses
