@@ -20,10 +20,23 @@ constexpr int PAD = TILE + 1;
 // 16 x 16 blocks.  MFMA f64 16x16x4 operand layout: lane l holds A[l%16][l/16]
 // and B[l/16][l%16]; accumulator value v of lane l is D[l/16 + 4*v][l%16]
 // (f64 layout: the four values of a lane are strided by 4 rows).
+// Tile of a workgroup.  Workgroups start in blockIdx order (x fastest); with a
+// k range that grows with the row (column) tile, the rows (columns) are taken
+// in reverse so that the heaviest tiles start first and the last ones to
+// start are short.  Each tile's arithmetic is the same either way.
+__device__ __forceinline__ void gemm_tile_of(const Gemm& g, int& bi, int& bj) {
+    bi = blockIdx.x;
+    bj = blockIdx.y;
+    if (g.kmode == K_BELOW_ROWTILE) bi = gridDim.x - 1 - bi;
+    else if (g.kmode == K_BELOW_COLTILE) bj = gridDim.y - 1 - bj;
+}
+
 __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
     __shared__ double As[GK][PAD];
     __shared__ double Bs[GK][PAD];
-    const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
+    int bi, bj;
+    gemm_tile_of(g, bi, bj);
+    const int z = blockIdx.z;
     if (g.lower_only && bi < bj) return;
     const int i0 = bi * TILE, j0 = bj * TILE;
     const double* A = g.A + z * g.strideA;
@@ -120,7 +133,9 @@ template <int TA, int TB>
 __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(Gemm g) {
     __shared__ double As[2][G2_BK][G2_PAD];
     __shared__ double Bs[2][G2_BK][G2_PAD];
-    const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
+    int bi, bj;
+    gemm_tile_of(g, bi, bj);
+    const int z = blockIdx.z;
     if (g.lower_only && bi < bj) return;
     const int i0 = bi * G2_T, j0 = bj * G2_T;
     const double* A = g.A + z * g.strideA;
