@@ -377,8 +377,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched-wp", type=int, default=32)
-    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--batched-wp", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serve-requests", type=int, default=10)
     ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-sweep", type=str, default="",
@@ -473,6 +473,7 @@ def main():
         log("batched leg")
         out["batched"] = run_batched(st, b, S, args.batched_wp, args.batched_steps)
         out["batched"]["fp64_vs_dropin"] = round(out["batched"]["fp64_value"] / value, 3)
+        out["batched"]["mixed_vs_dropin"] = round(out["batched"]["mixed_value"] / value, 3)
         if args.serve_requests > 0:
             log("serve leg")
             out["serve"] = run_serve(args.serve_requests, args.serve_rate)

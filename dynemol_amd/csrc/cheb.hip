@@ -880,8 +880,10 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
         s ^= 1;
     }
     // D^T layout: f64 lane l, value v -> column (l >> 4) + 4v, orbital l & 15;
-    //             f32 lane l, value v -> column 4 (l >> 4) + v, orbital l & 15
-    double* Pz = P + (long)z * C * a.ld;
+    //             f32 lane l, value v -> column 4 (l >> 4) + v, orbital l & 15.
+    // PREC 1 stores its FP32 slice sums as FP32 (exact, half the bytes).
+    using part_t = typename std::conditional<PREC == 1, float, double>::type;
+    part_t* Pz = reinterpret_cast<part_t*>(P) + (long)z * C * a.ld;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -890,11 +892,13 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + p * 16 + (lane & 15);
                 const int gc = c0 + wc + q * 16 + (PREC != 1 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
-                if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = (double)acc[p][q][v];
+                if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = (part_t)acc[p][q][v];
             }
 }
 
-__global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const double* __restrict__ P, int S) {
+// slices added in order, in FP64 (PT: how the slices were stored)
+template <typename PT>
+__global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const PT* __restrict__ P, int S) {
     const long n = (long)a.N * a.R;
     const long e = blockIdx.x * 256L + threadIdx.x;
     if (e >= n) return;
@@ -902,9 +906,9 @@ __global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const d
     const long C = 2L * a.R;
     cplx y{0.0, 0.0};
     for (int z = 0; z < S; ++z) {
-        const double* Pz = P + (long)z * C * a.ld;
-        y.re += Pz[(2L * r) * a.ld + i];
-        y.im += Pz[(2L * r + 1) * a.ld + i];
+        const PT* Pz = P + (long)z * C * a.ld;
+        y.re += (double)Pz[(2L * r) * a.ld + i];
+        y.im += (double)Pz[(2L * r + 1) * a.ld + i];
     }
     term_epilogue(a, i, r, y);
 }
@@ -919,7 +923,11 @@ void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st) {
     else if (prec == 1) hipLaunchKernelGGL((gemm3_kernel<1>), grid, dim3(256), 0, st, a, P, KS);
     else hipLaunchKernelGGL((gemm3_kernel<2>), grid, dim3(256), 0, st, a, P, KS);
     const long n = (long)a.N * a.R;
-    hipLaunchKernelGGL(gemm3_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, P, Sr);
+    const dim3 rg((unsigned)((n + 255) / 256));
+    if (prec == 1)
+        hipLaunchKernelGGL((gemm3_reduce_kernel<float>), rg, dim3(256), 0, st, a, reinterpret_cast<const float*>(P), Sr);
+    else
+        hipLaunchKernelGGL((gemm3_reduce_kernel<double>), rg, dim3(256), 0, st, a, (const double*)P, Sr);
 }
 
 static int g_gemm_bn = 0;  // 0: choose by size; 16 or 32 forces the tile width
