@@ -211,7 +211,7 @@ def _exact(S, h, psi0, t):
     return C @ (np.exp(-1j * E * t / hip.H_BAR) * (C.T @ (S @ psi0)))
 
 
-def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
+def run_many_systems(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
     """Many systems in one batch: C/H/N/O clusters, N log-uniform in [256, 2048],
     4 wavepackets each (random S-normalised superpositions of 8 frontier orbitals),
     grouped launches per Chebyshev term; sampled systems checked against the exact
@@ -242,7 +242,7 @@ def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
             systems.append(s)
             init.append(cs)
             if (i + 1) % 32 == 0:
-                log(f"config3: {i + 1} systems built")
+                log(f"many systems: {i + 1} systems built")
             mats.append((S, h) if i < check else None)
         out["setup_s"] = round(time.perf_counter() - t0, 2)
         out["n_min"] = min(s.N for s in systems)
@@ -257,7 +257,7 @@ def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
             batch.propagate(t, t + DT_PS)
             t += DT_PS
             if (step + 2) % 100 == 0:
-                log(f"config3: step {step + 2}")
+                log(f"many systems: step {step + 2}")
         el = time.perf_counter() - t0
         out["wavepacket_steps_per_s"] = round(n_sys * n_wp * (steps - 1) / el, 1)
         out["ms_per_step"] = round(1e3 * el / (steps - 1), 3)
@@ -280,17 +280,17 @@ def run_config3(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
     return out
 
 
-def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
+def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100):
     """One N = 8192 C/H/O interface system, n_wp random S-normalised wavepackets,
     FP64 matrix-core terms against FP32 products (mixed) and against the
     FP32-stored operator with FP64 products: population, norm and energy
     differences after each block of steps."""
     from dynemol_amd import hip
     from dynemol_amd.structures import interface_cluster
-    log("config4: generating the interface cluster")
+    log("precision study: generating the interface cluster")
     st = interface_cluster(seed=seed)
     b = st.basis()
-    log(f"config4: {st.n_atoms} atoms, N={b.N}")
+    log(f"precision study: {st.n_atoms} atoms, N={b.N}")
     rng = np.random.default_rng(seed)
     out = {"n_orbitals": b.N, "atoms": st.n_atoms, "wavepackets": n_wp, "rhs_columns": 2 * n_wp, "steps": steps}
     with hip.Context(0) as ctx:
@@ -300,7 +300,7 @@ def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
             s.set_precision(mode)
             s.build(st.coords)
-            log(f"config4 {name}: operators built, bounds {s.bounds()[:2]}")
+            log(f"precision study {name}: operators built, bounds {s.bounds()[:2]}")
             if mode == hip.DH_PREC_FP64:
                 S = s.matrix(hip.DH_MAT_S)
                 K = rng.normal(size=(b.N, n_wp)) + 1j * rng.normal(size=(b.N, n_wp))
@@ -309,7 +309,7 @@ def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
                 kets = [K[:, w] / nrm[w] for w in range(n_wp)]
                 bras = [SK[:, w] / nrm[w] for w in range(n_wp)]
                 del S, SK
-                log("config4: initial states ready")
+                log("precision study: initial states ready")
             for w in range(n_wp):
                 s.set_wavepacket(w, bras[w], kets[w])
             t, el, hist = 0.0, 0.0, []
@@ -319,7 +319,7 @@ def run_config4(steps=1000, seed=4, n_wp=64, check_every=100):
                 el += time.perf_counter() - t0
                 t += DT_PS
                 if (step + 1) % 10 == 0:
-                    log(f"config4 {name}: step {step + 1}, {s.stats()['terms']} terms")
+                    log(f"precision study {name}: step {step + 1}, {s.stats()['terms']} terms")
                 if (step + 1) % check_every == 0:
                     obs = [s.observables(w) for w in range(n_wp)]
                     hist.append((np.array([o[1] for o in obs]), np.array([o[2] for o in obs])))
@@ -387,10 +387,11 @@ def main():
     ap.add_argument("--term-variant", type=int, default=-1,
                     help="force one matvec variant instead of the autotuner's choice (for profiling: a "
                          "kernel tracer adds per-dispatch cost that skews the autotuner's timings)")
-    ap.add_argument("--config3", type=int, default=0, metavar="STEPS",
+    ap.add_argument("--many-systems", type=int, default=0, metavar="STEPS",
                     help="only run the 256-system batch case for STEPS steps and print its JSON line")
-    ap.add_argument("--config4", type=int, default=0, metavar="STEPS",
-                    help="only run the N=8192 FP64-vs-mixed precision case for STEPS steps and print its JSON line")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="only run the N=8192, 64-wavepacket FP64 / mixed / FP32-stored precision case for STEPS "
+                         "steps and print its JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB (5 launches), the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -398,11 +399,11 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
-    if args.config3:
-        print(json.dumps(run_config3(steps=args.config3)), flush=True)
+    if args.many_systems:
+        print(json.dumps(run_many_systems(steps=args.many_systems)), flush=True)
         return
-    if args.config4:
-        print(json.dumps(run_config4(steps=args.config4)), flush=True)
+    if args.precision_study:
+        print(json.dumps(run_precision_study(steps=args.precision_study)), flush=True)
         return
     if args.serve_sweep:
         for lam in (float(x) for x in args.serve_sweep.split(",")):
