@@ -58,7 +58,7 @@ struct dh_system {
     int split_max = 1;
     // Lanczos / Bessel
     cplx *lv = nullptr, *lvp = nullptr, *lw = nullptr;
-    double *lal = nullptr, *lbe = nullptr, *bounds = nullptr, *J = nullptr;
+    double *lal = nullptr, *lbe = nullptr, *J = nullptr;
     cplx* coef = nullptr;
     int* kout = nullptr;
     // observables
@@ -175,7 +175,7 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out, s->stage,
-                    s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->bounds, s->J, s->coef,
+                    s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -372,7 +372,6 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     SALLOC(s->lw, (size_t)ld);
     SALLOC(s->lal, LANCZOS_MAX + 2);
     SALLOC(s->lbe, LANCZOS_MAX + 2);
-    SALLOC(s->bounds, 8);
     SALLOC(s->J, KMAX_ALLOC + 2);
     SALLOC(s->coef, KMAX_ALLOC + 2);
     SALLOC(s->kout, 4);
@@ -450,10 +449,12 @@ static int compute_bounds(dh_system* s) {
         launch_matvec(a, st);
         launch_lanczos_step(s->N, s->lv, s->lvp, s->lw, s->lal, s->lbe, j, st);
     }
-    launch_lanczos_bounds(s->lal, s->lbe, m, s->rel_margin, s->bounds, st);
     DH_HIP(hipGetLastError());
-    DH_HIP(hipMemcpyAsync(s->bounds_h, s->bounds, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+    double al[LANCZOS_MAX], be[LANCZOS_MAX + 1];
+    DH_HIP(hipMemcpyAsync(al, s->lal, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    DH_HIP(hipMemcpyAsync(be, s->lbe, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, st));
     DH_HIP(hipStreamSynchronize(st));
+    lanczos_bounds_host(al, be, m, s->rel_margin, s->bounds_h);
     s->emin = s->bounds_h[0];
     s->emax = s->bounds_h[1];
     s->widen = 1.0;

@@ -19,6 +19,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <cmath>
 #include <type_traits>
 #include <utility>
 #include <mutex>
@@ -1657,7 +1658,7 @@ __global__ void lanczos_start_kernel(int N, cplx* v, cplx* vprev, unsigned seed)
 }
 
 // Sturm count: number of eigenvalues of the m x m tridiagonal below x.
-__device__ int sturm_count(const double* al, const double* be, int m, double x) {
+static int sturm_count(const double* al, const double* be, int m, double x) {
     int cnt = 0;
     double q = 1.0;
     for (int k = 0; k < m; ++k) {
@@ -1672,13 +1673,15 @@ __device__ int sturm_count(const double* al, const double* be, int m, double x) 
 // Extreme Ritz values by bisection, residual of each extreme Ritz pair
 // (|beta_m s_m|, s from inverse iteration on the tridiagonal), and the
 // enclosing interval [emin, emax] = [theta_min - margin_lo, theta_max + margin_hi].
-__global__ void lanczos_bounds_kernel(const double* al, const double* be, int m, double rel_margin, double* bounds) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Runs on the host: the m <= 120 recurrence coefficients come back with one
+// copy, and the serial bisection takes microseconds there against ~0.6 ms on
+// one GPU thread.
+void lanczos_bounds_host(const double* al, const double* be, int m, double rel_margin, double* bounds) {
     double lo = 1e300, hi = -1e300;
     for (int k = 0; k < m; ++k) {
-        const double r = (k > 0 ? fabs(be[k]) : 0.0) + (k + 1 < m ? fabs(be[k + 1]) : 0.0);
-        lo = fmin(lo, al[k] - r);
-        hi = fmax(hi, al[k] + r);
+        const double r = (k > 0 ? std::fabs(be[k]) : 0.0) + (k + 1 < m ? std::fabs(be[k + 1]) : 0.0);
+        lo = std::fmin(lo, al[k] - r);
+        hi = std::fmax(hi, al[k] + r);
     }
     double th[2];
     for (int e = 0; e < 2; ++e) {
@@ -1687,33 +1690,33 @@ __global__ void lanczos_bounds_kernel(const double* al, const double* be, int m,
         for (int it = 0; it < 200; ++it) {
             const double mid = 0.5 * (a + b);
             if (sturm_count(al, be, m, mid) >= target) b = mid; else a = mid;
-            if (b - a <= 1e-13 * fmax(1.0, fabs(mid))) break;
+            if (b - a <= 1e-13 * std::fmax(1.0, std::fabs(mid))) break;
         }
         th[e] = 0.5 * (a + b);
     }
     double res[2];
     for (int e = 0; e < 2; ++e) {
         // inverse iteration with a slightly shifted theta on the tridiagonal
-        const double shift = th[e] + (e == 0 ? -1e-10 : 1e-10) * fmax(1.0, fabs(th[e]));
+        const double shift = th[e] + (e == 0 ? -1e-10 : 1e-10) * std::fmax(1.0, std::fabs(th[e]));
         double s[128], dd[128], uu[128];
         for (int k = 0; k < m; ++k) s[k] = 1.0;
         for (int it = 0; it < 3; ++it) {
             // LU of (T - shift I) without pivoting (Thomas algorithm)
             dd[0] = al[0] - shift;
             for (int k = 1; k < m; ++k) {
-                const double l = be[k] / (fabs(dd[k - 1]) > 1e-300 ? dd[k - 1] : 1e-300);
+                const double l = be[k] / (std::fabs(dd[k - 1]) > 1e-300 ? dd[k - 1] : 1e-300);
                 dd[k] = al[k] - shift - l * be[k];
                 uu[k] = l;
             }
             for (int k = 1; k < m; ++k) s[k] -= uu[k] * s[k - 1];
-            s[m - 1] /= (fabs(dd[m - 1]) > 1e-300 ? dd[m - 1] : 1e-300);
-            for (int k = m - 2; k >= 0; --k) s[k] = (s[k] - be[k + 1] * s[k + 1]) / (fabs(dd[k]) > 1e-300 ? dd[k] : 1e-300);
+            s[m - 1] /= (std::fabs(dd[m - 1]) > 1e-300 ? dd[m - 1] : 1e-300);
+            for (int k = m - 2; k >= 0; --k) s[k] = (s[k] - be[k + 1] * s[k + 1]) / (std::fabs(dd[k]) > 1e-300 ? dd[k] : 1e-300);
             double nrm = 0.0;
             for (int k = 0; k < m; ++k) nrm += s[k] * s[k];
-            nrm = sqrt(nrm);
+            nrm = std::sqrt(nrm);
             for (int k = 0; k < m; ++k) s[k] /= nrm;
         }
-        res[e] = fabs(be[m] * s[m - 1]);
+        res[e] = std::fabs(be[m] * s[m - 1]);
     }
     const double width = th[1] - th[0];
     bounds[0] = th[0] - res[0] - rel_margin * width;
@@ -1924,10 +1927,6 @@ void launch_lanczos_start(int N, cplx* v, cplx* vprev, unsigned seed, hipStream_
 void launch_lanczos_step(int N, cplx* v, cplx* vprev, const cplx* w, double* alpha, double* beta, int j,
                          hipStream_t st) {
     hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, st, N, v, vprev, w, alpha, beta, j);
-}
-void launch_lanczos_bounds(const double* al, const double* be, int m, double rel_margin, double* bounds,
-                           hipStream_t st) {
-    hipLaunchKernelGGL(lanczos_bounds_kernel, dim3(1), dim3(64), 0, st, al, be, m, rel_margin, bounds);
 }
 void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k,
                    hipStream_t st) {

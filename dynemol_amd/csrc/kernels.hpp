@@ -110,8 +110,9 @@ void launch_zero_cplx(cplx* p, long n, hipStream_t st);
 void launch_lanczos_start(int N, cplx* v, cplx* vprev, unsigned seed, hipStream_t st);
 void launch_lanczos_step(int N, cplx* v, cplx* vprev, const cplx* w, double* alpha, double* beta, int j,
                          hipStream_t st);
-void launch_lanczos_bounds(const double* al, const double* be, int m, double rel_margin, double* bounds,
-                           hipStream_t st);
+// extreme-eigenvalue bounds of the Lanczos tridiagonal (al[0..m-1], be[1..m]),
+// on the host: bounds = [emin, emax, theta_min, theta_max, res_min, res_max]
+void lanczos_bounds_host(const double* al, const double* be, int m, double rel_margin, double* bounds);
 void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k,
                    hipStream_t st);
 void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi, const cplx* hk,
