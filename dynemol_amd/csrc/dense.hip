@@ -129,13 +129,10 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(Gemm g) {
 // as in the 64 x 64 kernel.
 constexpr int G2_T = 128, G2_BK = 16, G2_PAD = G2_T + 4;
 
+// One 128 x 128 output tile (bi, bj) of batch z; the LDS stages are the caller's.
 template <int TA, int TB>
-__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(Gemm g) {
-    __shared__ double As[2][G2_BK][G2_PAD];
-    __shared__ double Bs[2][G2_BK][G2_PAD];
-    int bi, bj;
-    gemm_tile_of(g, bi, bj);
-    const int z = blockIdx.z;
+__device__ __forceinline__ void gemm128_tile(const Gemm& g, int bi, int bj, int z, double (*As)[G2_BK][G2_PAD],
+                                             double (*Bs)[G2_BK][G2_PAD]) {
     if (g.lower_only && bi < bj) return;
     const int i0 = bi * G2_T, j0 = bj * G2_T;
     const double* A = g.A + z * g.strideA;
@@ -232,15 +229,54 @@ __global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(Gemm g) {
             }
 }
 
+// A workgroup computes one tile, or with g.pair two tiles whose k ranges add up
+// to the same length (row tiles i and n-1-i of a triangular k range, or the
+// same for columns): every workgroup then does equal work, which a one-tile
+// grid cannot give when the heavy tiles land on a few XCDs.
+template <int TA, int TB>
+__global__ __launch_bounds__(256, 2) void gemm128_f64_kernel(Gemm g) {
+    __shared__ double As[2][G2_BK][G2_PAD];
+    __shared__ double Bs[2][G2_BK][G2_PAD];
+    const int z = blockIdx.z;
+    int bi, bj;
+    gemm_tile_of(g, bi, bj);
+    int npass = 1;
+    if (g.pair) {
+        const int rows = g.pair == 1;
+        const int n = rows ? (g.M + G2_T - 1) / G2_T : (g.N + G2_T - 1) / G2_T;
+        bi = blockIdx.x;
+        bj = blockIdx.y;
+        npass = (rows ? n - 1 - bi : n - 1 - bj) != (rows ? bi : bj) ? 2 : 1;
+    }
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+        if (pass == 1) {
+            __syncthreads();  // the first tile's LDS readers are done
+            if (g.pair == 1) bi = (g.M + G2_T - 1) / G2_T - 1 - bi;
+            else bj = (g.N + G2_T - 1) / G2_T - 1 - bj;
+        }
+        gemm128_tile<TA, TB>(g, bi, bj, z, As, Bs);
+    }
+}
+
 void launch_gemm(const Gemm& g, hipStream_t st) {
     if (g.M <= 0 || g.N <= 0) return;
     const unsigned nz = g.batch > 0 ? g.batch : 1;
     if (g.M >= 2 * G2_T && g.N >= 2 * G2_T) {
+        Gemm p = g;
         dim3 grid((g.M + G2_T - 1) / G2_T, (g.N + G2_T - 1) / G2_T, nz);
-        if (!g.transA && !g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<0, 0>), grid, dim3(256), 0, st, g);
-        else if (!g.transA) hipLaunchKernelGGL((gemm128_f64_kernel<0, 1>), grid, dim3(256), 0, st, g);
-        else if (!g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<1, 0>), grid, dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((gemm128_f64_kernel<1, 1>), grid, dim3(256), 0, st, g);
+        p.pair = 0;
+        if (!g.lower_only && (g.kmode == K_BELOW_ROWTILE || g.kmode == K_FROM_ROWTILE)) {
+            p.pair = 1;
+            grid.x = (grid.x + 1) / 2;
+        } else if (!g.lower_only && (g.kmode == K_BELOW_COLTILE || g.kmode == K_FROM_COLTILE)) {
+            p.pair = 2;
+            grid.y = (grid.y + 1) / 2;
+        }
+        if (!g.transA && !g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<0, 0>), grid, dim3(256), 0, st, p);
+        else if (!g.transA) hipLaunchKernelGGL((gemm128_f64_kernel<0, 1>), grid, dim3(256), 0, st, p);
+        else if (!g.transB) hipLaunchKernelGGL((gemm128_f64_kernel<1, 0>), grid, dim3(256), 0, st, p);
+        else hipLaunchKernelGGL((gemm128_f64_kernel<1, 1>), grid, dim3(256), 0, st, p);
         return;
     }
     dim3 grid((g.M + TILE - 1) / TILE, (g.N + TILE - 1) / TILE, nz);

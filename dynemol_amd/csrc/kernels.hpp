@@ -44,6 +44,7 @@ struct Gemm {
     int kmode;       // KMode: restrict k to where a triangular operand is nonzero
     int lower_only;  // compute only tiles with row-tile >= col-tile
     int batch;
+    int pair;        // set by launch_gemm: 1 = pair row tiles i, n-1-i; 2 = column tiles
 };
 void launch_gemm(const Gemm& g, hipStream_t st);
 
