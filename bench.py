@@ -280,19 +280,21 @@ def run_many_systems(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
     return out
 
 
-def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100):
-    """One N = 8192 C/H/O interface system, n_wp random S-normalised wavepackets,
-    FP64 matrix-core terms against FP32 products (mixed) and against the
-    FP32-stored operator with FP64 products: population, norm and energy
-    differences after each block of steps."""
+def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100, system="interface"):
+    """n_wp random S-normalised wavepackets on one fixed system, FP64
+    matrix-core terms against FP32 products (mixed) and against the FP32-stored
+    operator with FP64 products: population, norm and energy differences after
+    each block of steps.  system: "interface" = the N = 8192 C/H/O interface
+    cluster; "tio2" = the N = 4096 TiO2 slab + C16H20 of the drop-in."""
     from dynemol_amd import hip
-    from dynemol_amd.structures import interface_cluster
-    log("precision study: generating the interface cluster")
-    st = interface_cluster(seed=seed)
+    from dynemol_amd.structures import interface_cluster, tio2_slab_adsorbate
+    log(f"precision study: generating the {system} system")
+    st = interface_cluster(seed=seed) if system == "interface" else tio2_slab_adsorbate(seed=2)
     b = st.basis()
     log(f"precision study: {st.n_atoms} atoms, N={b.N}")
     rng = np.random.default_rng(seed)
-    out = {"n_orbitals": b.N, "atoms": st.n_atoms, "wavepackets": n_wp, "rhs_columns": 2 * n_wp, "steps": steps}
+    out = {"system": system, "n_orbitals": b.N, "atoms": st.n_atoms, "wavepackets": n_wp, "rhs_columns": 2 * n_wp,
+           "steps": steps}
     with hip.Context(0) as ctx:
         res = {}
         names = {hip.DH_PREC_FP64: "fp64", hip.DH_PREC_MIXED: "mixed", hip.DH_PREC_F32_STORAGE: "f32_storage"}
@@ -392,6 +394,8 @@ def main():
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="only run the N=8192, 64-wavepacket FP64 / mixed / FP32-stored precision case for STEPS "
                          "steps and print its JSON line")
+    ap.add_argument("--precision-system", choices=("interface", "tio2"), default="interface",
+                    help="system of --precision-study: the N=8192 interface cluster or the N=4096 TiO2 drop-in system")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB (5 launches), the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -403,7 +407,7 @@ def main():
         print(json.dumps(run_many_systems(steps=args.many_systems)), flush=True)
         return
     if args.precision_study:
-        print(json.dumps(run_precision_study(steps=args.precision_study)), flush=True)
+        print(json.dumps(run_precision_study(steps=args.precision_study, system=args.precision_system)), flush=True)
         return
     if args.serve_sweep:
         for lam in (float(x) for x in args.serve_sweep.split(",")):
