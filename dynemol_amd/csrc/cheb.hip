@@ -1060,10 +1060,9 @@ bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2
 
 
 // x and H are read-only here and restrict-qualified, so wave-uniform x reads
-// become scalar loads.  MT is H's storage type: double, or float for the
-// FP32-stored operator (widened on load; all arithmetic stays FP64).
-template <int R, typename MT>
-__global__ __launch_bounds__(256) void sym_tiles_kernel(const MT* __restrict__ Mg, const cplx* __restrict__ xg,
+// become scalar loads.
+template <int R>
+__global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict__ Mg, const cplx* __restrict__ xg,
                                                         cplx* __restrict__ part, int ld, int N, int x_stride,
                                                         int x_off, int S) {
     constexpr int HP = TILE + 1;  // padded LDS column
@@ -1096,7 +1095,7 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const MT* __restrict__ M
     auto fetch = [&](int J) {
         const int j0 = J * TILE + 16 * w;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) hn[s] = (double)Mg[(long)(j0 + s) * ld + row];
+        for (int s = 0; s < 16; ++s) hn[s] = Mg[(long)(j0 + s) * ld + row];
     };
     auto xload = [&](int j, double* xr, double* xi) {  // wave-uniform row j of x
         const cplx* p = xg + (long)min(j, N - 1) * x_stride + x_off;
@@ -1264,14 +1263,11 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecAr
     if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
 }
 
-template <int R, typename MT = double>
+template <int R>
 static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int nb = a.ld / TILE;
     const int S = sym_seg_for(nb);
-    const MT* M;
-    if constexpr (std::is_same<MT, float>::value) M = a.M32;
-    else M = a.M;
-    hipLaunchKernelGGL((sym_tiles_kernel<R, MT>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, M, a.x,
+    hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
                        a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
@@ -1301,8 +1297,8 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
 // in groups of 4 real columns (lane l: column 4h + (l & 3)).  For R <= 4 this
 // does no discarded work: ceil(2R / 4) products per k-step of 16 cycles each,
 // against one 64-cycle 16x16x4 whose rows >= 2R are discarded.
-template <int R, int MF>
-__global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
+template <int R, int MF, typename MT = double>
+__global__ __launch_bounds__(256) void symm_tiles_kernel(const MT* __restrict__ Mg, const double* __restrict__ xg,
                                                          double* __restrict__ part, int ld, int N, int x_stride,
                                                          int x_off, int S) {
     constexpr int HP = TILE + 1;
@@ -1340,7 +1336,7 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restric
     auto fetch = [&](int J) {
         const int j0 = J * TILE;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) hn[c] = Mg[(long)(j0 + 4 * c + lk) * ld + i0 + 16 * w + lr];
+        for (int c = 0; c < 16; ++c) hn[c] = (double)Mg[(long)(j0 + 4 * c + lk) * ld + i0 + 16 * w + lr];
         xfetch(j0, xn);
     };
     {
@@ -1434,11 +1430,15 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const double* __restric
     }
 }
 
-template <int R, int MF>
+// MT = float: H's FP32 copy (a.M32), widened to FP64 on load (precision mode 2)
+template <int R, int MF, typename MT = double>
 static void launch_symm(const MatvecArgs& a, hipStream_t st) {
     const int nb = a.ld / TILE;
     const int S = sym_seg_for(nb, 3);
-    hipLaunchKernelGGL((symm_tiles_kernel<R, MF>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
+    const MT* M;
+    if constexpr (std::is_same<MT, float>::value) M = a.M32;
+    else M = a.M;
+    hipLaunchKernelGGL((symm_tiles_kernel<R, MF, MT>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off, S);
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
@@ -1578,18 +1578,19 @@ void launch_matvec(const MatvecArgs& a, hipStream_t st) {
 }
 
 // Term on the FP32-stored operator a.M32 with FP64 arithmetic: the symmetric
-// half-storage kernel for R <= 8, else the split-K GEMM with FP64 products.
-// One fixed kernel per shape, so results are reproducible run to run.
+// half-storage kernel with 4x4x4 matrix-core products for R <= 8, else the
+// split-K GEMM with FP64 products.  One fixed kernel per shape, so results are
+// reproducible run to run.
 void launch_term_f32h(const MatvecArgs& a, hipStream_t st) {
     if (!sym_supported(a.R, a.tri)) {
         launch_gemm3(a, 2, reinterpret_cast<double*>(a.partial), st);
         return;
     }
     switch (a.R) {
-        case 1: launch_sym<1, float>(a, st); break;
-        case 2: launch_sym<2, float>(a, st); break;
-        case 4: launch_sym<4, float>(a, st); break;
-        default: launch_sym<8, float>(a, st); break;
+        case 1: launch_symm<1, 1, float>(a, st); break;
+        case 2: launch_symm<2, 1, float>(a, st); break;
+        case 4: launch_symm<4, 1, float>(a, st); break;
+        default: launch_symm<8, 1, float>(a, st); break;
     }
 }
 
