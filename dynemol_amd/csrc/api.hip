@@ -202,7 +202,7 @@ static long partial_cplx_for(int N, int n_wp) {
     long n = (long)matvec_split_for(N, 1) * ld * 16;
     const int rs[3] = {2 * n_wp, n_wp, 1};
     for (int R : rs) {
-        if (sym_supported(R, TRI_FULL)) n = std::max(n, sym_partial_cplx(ld, R));
+        if (sym_supported(R, TRI_FULL)) n = std::max(n, std::max(sym_partial_cplx(ld, R), sym2_partial_cplx(ld, R)));
         n = std::max(n, (gemm3_partial_doubles(N, ld, R) + 1) / 2);
     }
     return n;
@@ -1181,7 +1181,8 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
     DALLOC(acc, (size_t)ld * R);
     DALLOC(coef, 8);
     DALLOC(partial, (size_t)std::max(std::max((long)split * ld * 16, sym_partial_cplx(ld, R <= 8 ? R : 8)),
-                                     (gemm3_partial_doubles(N, ld, R) + 1) / 2));
+                                     std::max(sym2_partial_cplx(ld, R <= 8 ? R : 8),
+                                              (gemm3_partial_doubles(N, ld, R) + 1) / 2)));
     DALLOC(counters, (size_t)ld);
     DH_HIP(hipMemset(M, 0, sizeof(double) * (size_t)ld * ld));
     DH_HIP(hipMemset(x, 0, sizeof(cplx) * (size_t)ld * R));

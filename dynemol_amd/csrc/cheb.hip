@@ -1444,12 +1444,191 @@ static void launch_symm(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
 
+// ------------------------------------------------------------------ symmetric, 2 x 2 super-tiles
+// Variant 19: the 4x4x4 matrix-core tile products of variant 18, with a
+// workgroup owning the super-tile (P, Q), Q <= P: block rows {2P, 2P+1} x
+// block columns {2Q, 2Q+1} of the lower triangle, up to four tiles, processed
+// one after the other with the next tile's loads in flight.  Row sums collect
+// over the two tiles of each block row and column sums over the two tiles of
+// each block column before they are written, so there are (up to) 2 + 2
+// partials per four tiles, against 1 + 1/S per tile for row segments.
+// Partials: Pcol[P^2 + J] (64 x R, J <= 2P), then Prow[2 (P (P + 1) / 2 + Q) + a].
+long sym2_partial_cplx(int ld, int R) {
+    const long nb = ld / TILE, np = (nb + 1) / 2;
+    return (np * np + np * (np + 1)) * TILE * R;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void symm2_tiles_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
+                                                          double* __restrict__ part, int ld, int N, int x_stride,
+                                                          int x_off) {
+    constexpr int HP = TILE + 1;
+    constexpr int C2 = 2 * R;
+    constexpr int XE = TILE * C2;
+    constexpr int XPT = (XE + 255) / 256;
+    constexpr int G = (C2 + 3) / 4;
+    __shared__ double Hs[TILE * HP];
+    __shared__ double xs[2 * XE];   // x of block columns 2Q, 2Q + 1
+    __shared__ double xis[2 * XE];  // x of block rows 2P, 2P + 1
+    const int nb = ld / TILE, npr = (nb + 1) / 2;
+    const int idx = blockIdx.x;
+    int P = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while (P * (P + 1) / 2 > idx) --P;
+    while ((P + 1) * (P + 2) / 2 <= idx) ++P;
+    const int Q = idx - P * (P + 1) / 2;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lr = lane & 15, lk = lane >> 4, xq = lane & 3;
+    const long xs2 = 2L * x_stride, xo2 = 2L * x_off;
+    const bool hasB = 2 * P + 1 < nb;
+    // valid tiles k = 2a + b: (2P + a, 2Q + b) inside the matrix and on or below the diagonal
+    const unsigned valid = 1u | ((Q < P ? 1u : 0u) << 1) | ((hasB ? 1u : 0u) << 2) | ((hasB ? 1u : 0u) << 3);
+    // x blocks (a missing block is clamped to the last one and never used); H's
+    // padding is [0 0; 0 I] and x rows >= N are read as row N - 1
+#pragma unroll
+    for (int blk = 0; blk < 2; ++blk) {
+        const int rI = min(2 * P + blk, nb - 1) * TILE, rJ = min(2 * Q + blk, nb - 1) * TILE;
+#pragma unroll
+        for (int u = 0; u < XPT; ++u) {
+            const int e = t + 256 * u;
+            if (XE % 256 == 0 || e < XE) {
+                xis[blk * XE + e] = xg[(long)min(rI + e / C2, N - 1) * xs2 + xo2 + e % C2];
+                xs[blk * XE + e] = xg[(long)min(rJ + e / C2, N - 1) * xs2 + xo2 + e % C2];
+            }
+        }
+    }
+    double hn[16];
+    auto fetch = [&](int k) {
+        const int i0 = (2 * P + (k >> 1)) * TILE, j0 = (2 * Q + (k & 1)) * TILE;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) hn[c] = Mg[(long)(j0 + 4 * c + lk) * ld + i0 + 16 * w + lr];
+    };
+    double yacc[2][G], zacc[2][G];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int h = 0; h < G; ++h) yacc[a][h] = zacc[a][h] = 0.0;
+    fetch(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((valid >> k) & 1u)) continue;  // uniform over the workgroup
+        const int a = k >> 1, b = k & 1;
+        const bool diag = 2 * P + a == 2 * Q + b;
+        double hc[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) hc[c] = hn[c];
+        __syncthreads();  // previous tile's LDS readers are done (and x staged, first time)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) Hs[(4 * c + lk) * HP + 16 * w + lr] = hc[c];
+        int kn = k + 1;
+        while (kn < 4 && !((valid >> kn) & 1u)) ++kn;
+        if (kn < 4) fetch(kn);
+        __syncthreads();  // tile staged
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                yacc[a][h] = __builtin_amdgcn_mfma_f64_4x4x4f64(
+                    hc[c], xs[b * XE + (4 * c + lk) * C2 + min(4 * h + xq, C2 - 1)], yacc[a][h], 0, 0, 0);
+        if (diag) continue;  // diagonal tile: the row sums cover it
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const double hv = Hs[(16 * w + lr) * HP + 4 * r + lk];
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                zacc[b][h] = __builtin_amdgcn_mfma_f64_4x4x4f64(
+                    hv, xis[a * XE + (4 * r + lk) * C2 + min(4 * h + xq, C2 - 1)], zacc[b][h], 0, 0, 0);
+        }
+    }
+    // D lane l: row (column) 16w + 4 ((l >> 2) & 3) + (l >> 4) of its block, real column 4h + (l & 3)
+    const int rowl = 16 * w + 4 * ((lane >> 2) & 3) + lk;
+    const long E = (long)TILE * R;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        if (!(Q < P || (b == 0 && hasB))) continue;  // no off-diagonal tile in column 2Q + b
+        double* pz = part + 2 * (((long)P * P + 2 * Q + b) * E + (long)rowl * R);
+#pragma unroll
+        for (int h = 0; h < G; ++h)
+            if (4 * h + xq < C2) pz[4 * h + xq] = zacc[b][h];
+    }
+    const long prow = (long)npr * npr * E;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !hasB) continue;
+        double* py = part + 2 * (prow + ((long)idx * 2 + a) * E + (long)rowl * R);
+#pragma unroll
+        for (int h = 0; h < G; ++h)
+            if (4 * h + xq < C2) py[4 * h + xq] = yacc[a][h];
+    }
+}
+
+// Phase 2 of variant 19: block B adds the row partials of super-tiles
+// (B / 2, 0..B / 2) (row B % 2 of each) and the column partials Pcol[P^2 + B]
+// for the super-tile rows P below B, in a fixed order, then the epilogue.
+template <int R>
+__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecArgs a) {
+    __shared__ cplx red[SYM_G * 64];
+    const int nb = a.ld / TILE, npr = (nb + 1) / 2;
+    const int B = blockIdx.x, chunk = blockIdx.y;
+    const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
+    const long E = (long)TILE * R;
+    const long prow = (long)npr * npr * E;
+    const int P = B / 2, arow = B % 2;
+    const int plo = (B % 2 == 0 && B + 1 < nb) ? B / 2 : B / 2 + 1;  // first super-tile row with a column partial
+    const int nr = P + 1;
+    const int L = nr + (npr - plo);
+    const int s0 = g * L / SYM_G, s1 = (g + 1) * L / SYM_G;
+    EpiOperands ep{};
+    if (t < 64) {
+        const int i = B * TILE + e / R;
+        if (i < a.N) ep = epilogue_operands(a, i, e % R);
+    }
+    double sr = 0.0, si = 0.0;
+    for (int base = s0; base < s1; base += SYM_MAXS) {
+        cplx p[SYM_MAXS];
+#pragma unroll
+        for (int u = 0; u < SYM_MAXS; ++u) {
+            const int src = min(base + u, s1 - 1);
+            const long P2 = plo + (src - nr);
+            const long off = src < nr ? prow + ((long)(P * (P + 1) / 2 + src) * 2 + arow) * E : (P2 * P2 + B) * E;
+            p[u] = a.partial[off + e];
+        }
+#pragma unroll
+        for (int u = 0; u < SYM_MAXS; ++u)
+            if (base + u < s1) {
+                sr += p[u].re;
+                si += p[u].im;
+            }
+    }
+    red[t] = cplx{sr, si};
+    __syncthreads();
+    if (t >= 64) return;
+    cplx y{0.0, 0.0};
+#pragma unroll
+    for (int gg = 0; gg < SYM_G; ++gg) {
+        y.re += red[gg * 64 + t].re;
+        y.im += red[gg * 64 + t].im;
+    }
+    const int i = B * TILE + e / R;
+    if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
+}
+
+template <int R>
+static void launch_symm2(const MatvecArgs& a, hipStream_t st) {
+    const int nb = a.ld / TILE, npr = (nb + 1) / 2;
+    hipLaunchKernelGGL((symm2_tiles_kernel<R>), dim3(npr * (npr + 1) / 2), dim3(256), 0, st, a.M,
+                       reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
+                       a.x_stride, a.x_off);
+    hipLaunchKernelGGL((sym2_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
+}
+
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
 // counters); 1..4 and 9..14 = column-block kernels of different shapes and LDS
 // schedules; 5..8 = two-phase split-K; 15 = split-K matrix-core GEMM (full
 // operator); 16 = symmetric half storage, vector FMAs; 17 = symmetric half
-// storage, matrix-core tile products (16x16x4); 18 = the same with 4x4x4 products.
-constexpr int MV_VARIANTS = 19;
+// storage, matrix-core tile products (16x16x4); 18 = the same with 4x4x4 products;
+// 19 = 4x4x4 products over 2 x 2 super-tiles.
+constexpr int MV_VARIANTS = 20;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -1512,6 +1691,19 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
+    if (variant == 19) {
+        if (!sym_supported(a.R, a.tri)) {
+            launch_matvec_variant(a, 1, st);
+            return;
+        }
+        switch (a.R) {
+            case 1: launch_symm2<1>(a, st); break;
+            case 2: launch_symm2<2>(a, st); break;
+            case 4: launch_symm2<4>(a, st); break;
+            default: launch_symm2<8>(a, st); break;
+        }
+        return;
+    }
     if (variant == 17 || variant == 18) {
         // symmetric kernel with matrix-core tile products (16x16x4 or 4x4x4);
         // other shapes use variant 1

@@ -84,6 +84,7 @@ struct MatvecArgs {
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
 long sym_partial_cplx(int ld, int R);
+long sym2_partial_cplx(int ld, int R);
 void launch_strided_copy(cplx* dst, long ds, const cplx* src, long ss, int n, hipStream_t st);
 bool sym_supported(int R, int tri);
 void launch_matvec(const MatvecArgs& a, hipStream_t st);
