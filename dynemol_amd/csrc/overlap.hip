@@ -44,12 +44,11 @@ __device__ void solap_dev(int na, int la, double za, int nb, int lb, double zb, 
 #define SMM(n, np) smatm[(n) * SQ_D + (np)]
     // h1f1 needs no clearing: level ngmax is filled for a = 1..namax, level j
     // for a = 1..min(namax, j - 1) from level j + 1, and every later read
-    // (j + 1, k + 2) has j + 1 <= min(namax, k + 1).
-    for (int i = 0; i < SQ_D * SQ_D; ++i) {
-        smat[i] = 0.0;
-        smataux[i] = 0.0;
-        smatm[i] = 0.0;
-    }
+    // (j + 1, k + 2) has j + 1 <= min(namax, k + 1).  Nor do smat, smataux and
+    // smatm: the basic integrals fill n = nmin..nmax, n' = n'min..n'max - (n -
+    // nmin), and every read of the L, L' and M recursions that follow lands on
+    // an entry written before it (checked over every n <= 5, l <= 2 shell
+    // pair in both zeta orders, all indices within the 12 x 12 tables).
     int nmin, nmax, npmin, npmax;
     double z, zp;
     const bool a_first = za >= zb;
@@ -97,6 +96,7 @@ __device__ void solap_dev(int na, int la, double za, int nb, int lb, double zb, 
 
     // basic integrals <N00|N'00>
     {
+        const bool mcopy = min(la, lb) > 0;
         const double zzpR = (z + zp) * R;
         const double zzpi = 1.0 / (z + zp);
         double zzpin = 1.0;
@@ -114,10 +114,14 @@ __device__ void solap_dev(int na, int la, double za, int nb, int lb, double zb, 
                     auxk = auxk * zzpR * rreali(n + np - k) * rreali(k + 2);
                 }
                 const double v = pi4 * expzR * dfact(n) * dfact(np) * sumk;
+                // smataux needs no copy (every read of it sees a recursion
+                // write), smatm only when an M step follows (min(la, lb) > 0)
                 if (a_first) {
-                    SM(n, np) = v; SA(n, np) = v; SMM(n, np) = v;
+                    SM(n, np) = v;
+                    if (mcopy) SMM(n, np) = v;
                 } else {
-                    SM(np, n) = v; SA(np, n) = v; SMM(np, n) = v;
+                    SM(np, n) = v;
+                    if (mcopy) SMM(np, n) = v;
                 }
                 zzpinp *= zzpi;
             }
