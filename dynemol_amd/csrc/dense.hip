@@ -150,6 +150,22 @@ __device__ __forceinline__ void gemm128_tile(const Gemm& g, int bi, int bj, int 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    // C <- C - A B (alpha = -1, beta = 1): C is read into the accumulators
+    // before the k loop, where its latency overlaps the operand loads, and the
+    // products are subtracted (A negated on its way out of LDS, exactly)
+    const bool cacc = g.c_acc != 0;
+    if (cacc) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int gi = i0 + wi + a * 16 + (lane & 15);
+                    const int gj = j0 + wj + b * 16 + (lane >> 4) + 4 * v;
+                    acc[a][b][v] = (gi < g.M && gj < g.N) ? C[gi + g.ldc * (long)gj] : 0.0;
+                }
+    }
 
     double ra[8], rb[8];
     auto gload = [&](int k0) {
@@ -199,7 +215,10 @@ __device__ __forceinline__ void gemm128_tile(const Gemm& g, int bi, int bj, int 
         for (int ks = 0; ks < G2_BK; ks += 4) {
             double av[4], bv[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[s][ks + (lane >> 4)][wi + a * 16 + (lane & 15)];
+            for (int a = 0; a < 4; ++a) {
+                const double x = As[s][ks + (lane >> 4)][wi + a * 16 + (lane & 15)];
+                av[a] = cacc ? -x : x;
+            }
 #pragma unroll
             for (int b = 0; b < 4; ++b) bv[b] = Bs[s][ks + (lane >> 4)][wj + b * 16 + (lane & 15)];
 #pragma unroll
@@ -223,8 +242,12 @@ __device__ __forceinline__ void gemm128_tile(const Gemm& g, int bi, int bj, int 
                 const int gj = j0 + wj + b * 16 + (lane >> 4) + 4 * v;
                 if (gi < g.M && gj < g.N) {
                     const long idx = gi + g.ldc * (long)gj;
-                    const double old = g.beta == 0.0 ? 0.0 : g.beta * C[idx];
-                    C[idx] = g.alpha * acc[a][b][v] + old;
+                    if (cacc) {
+                        C[idx] = acc[a][b][v];
+                    } else {
+                        const double old = g.beta == 0.0 ? 0.0 : g.beta * C[idx];
+                        C[idx] = g.alpha * acc[a][b][v] + old;
+                    }
                 }
             }
 }
@@ -266,6 +289,7 @@ void launch_gemm(const Gemm& g, hipStream_t st) {
         Gemm p = g;
         dim3 grid((g.M + G2_T - 1) / G2_T, (g.N + G2_T - 1) / G2_T, nz);
         p.pair = 0;
+        p.c_acc = g.alpha == -1.0 && g.beta == 1.0;
         if (!g.lower_only && (g.kmode == K_BELOW_ROWTILE || g.kmode == K_FROM_ROWTILE)) {
             p.pair = 1;
             grid.x = (grid.x + 1) / 2;

@@ -45,6 +45,7 @@ struct Gemm {
     int lower_only;  // compute only tiles with row-tile >= col-tile
     int batch;
     int pair;        // set by launch_gemm: 1 = pair row tiles i, n-1-i; 2 = column tiles
+    int c_acc;       // set by launch_gemm for alpha = -1, beta = 1: C starts in the accumulators
 };
 void launch_gemm(const Gemm& g, hipStream_t st);
 
