@@ -925,6 +925,10 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
         if (s->timing) DH_HIP(hipEventRecord(s->ev1, st));
         std::swap(s->phi, s->acc);  // phi <- new state; acc keeps the old one
         launch_column_norms(s->N, R, s->phi, s->norms1, st);
+        // the outputs go out with the norms, so one host wait covers the step;
+        // an attempt that fails the drift check below recomputes them
+        if (s->precision == 1) launch_renorm_columns(s->phi, s->N, R, s->norms0, s->norms1, st);
+        launch_outputs(s);
         DH_HIP(hipMemcpyAsync(n0.data(), s->norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         DH_HIP(hipMemcpyAsync(n1.data(), s->norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         DH_HIP(hipStreamSynchronize(st));
@@ -941,9 +945,6 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
             s->stats[6] += s->K;
         }
         if (drift < drift_tol) {
-            if (s->precision == 1) launch_renorm_columns(s->phi, s->N, R, s->norms0, s->norms1, st);
-            launch_outputs(s);
-            DH_HIP(hipStreamSynchronize(st));
             s->stats[0] = s->K + 1;
             s->stats[1] += 1;
             s->stats[2] += s->K + 3;
@@ -955,6 +956,9 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
         s->widen *= 1.25;
         s->stats[3] += 1;
     }
+    // outputs of the restored (pre-step) state, as before the call
+    launch_outputs(s);
+    DH_HIP(hipStreamSynchronize(st));
     dh::set_error("dh_propagate: Chebyshev series lost the norm (drift %g) after widening the spectral bounds",
                   s->stats[4]);
     return DH_ERR_DIVERGED;
