@@ -92,6 +92,9 @@ struct dh_system {
     // FP64 products and sums
     int precision = 0;
     float* H32 = nullptr;     // FP32 copy of H_orth for modes 1 and 2
+    // mode 1: FP32 copies of the term inputs (the step's initial state, then
+    // each T_k as its epilogue writes it), alternating between two buffers
+    float* x32[2] = {nullptr, nullptr};
     bool h32_ready = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -170,6 +173,8 @@ extern "C" int dh_system_destroy(dh_system* s) {
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
     if (s->H32) hipFree(s->H32);
+    for (float* p : s->x32)
+        if (p) hipFree(p);
     for (auto& g : s->graphs)
         if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
@@ -779,7 +784,14 @@ static int series_launches(dh_system* s) {
     a.coef = s->coef;
     a.M32 = s->H32;
     const int prec = s->precision;
-    auto term = [&](const MatvecArgs& args) {
+    // mode 1: the FP32 GEMM reads FP32 copies of its input; the step's state
+    // is converted once, then each term's epilogue writes the next one's
+    if (prec == 1) launch_to_f32(reinterpret_cast<const double*>(s->phi), s->x32[0], (long)s->ld * 2 * R, st);
+    auto term = [&](MatvecArgs args) {
+        if (prec == 1) {
+            args.x32 = s->x32[(args.k - 1) & 1];
+            args.out32 = s->x32[args.k & 1];
+        }
         if (prec == 1) launch_gemm3(args, 1, reinterpret_cast<double*>(s->partial), st);
         else if (prec == 2) launch_term_f32h(args, st);
         else launch_matvec(args, st);
@@ -1231,6 +1243,9 @@ extern "C" int dh_set_precision(dh_system* s, int mode) {
               "dh_set_precision: mode must be 0 (FP64), 1 (mixed) or 2 (FP32-stored operator)");
     DH_HIP(hipSetDevice(s->ctx->device));
     if (mode != 0 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
+    if (mode == 1)
+        for (float*& p : s->x32)
+            if (!p) DALLOC(p, (size_t)s->ld * 4 * s->W);
     s->precision = mode;
     if (mode != 0 && s->ops_ready && !s->h32_ready) {
         launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);

@@ -728,6 +728,10 @@ __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r,
         a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
     }
     a.out[oi] = T;
+    if (a.out32) {  // the next term's FP32 operand: the value its GEMM would round on load
+        a.out32[2 * oi] = (float)T.re;
+        a.out32[2 * oi + 1] = (float)T.im;
+    }
 }
 
 // The epilogue with its operands loaded ahead (they do not depend on y).
@@ -809,6 +813,7 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wi = (w >> 1) * 64, wc = (w & 1) * 64;
     const double* X = reinterpret_cast<const double*>(a.x);
+    const float* X32 = PREC == 1 ? a.x32 : nullptr;  // FP32 copy of x when the caller keeps one
     const long xs = 2L * a.x_stride, xo = 2L * a.x_off;
     using acc_t = typename std::conditional<PREC == 1, v4f, v4d>::type;
     acc_t acc[4][4];
@@ -837,9 +842,9 @@ __global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __rest
         for (int r = 0; r < LPT; ++r) {
             const int cc = t & 127, kk = (t >> 7) + 2 * r;
             const int gc = c0 + cc, gk = k0 + kk;
-            double v = 0.0;
-            if (gc < C && gk < khi) v = X[gk * xs + xo + gc];
-            rb[r] = (T)v;
+            T v = (T)0;
+            if (gc < C && gk < khi) v = X32 ? (T)X32[gk * xs + xo + gc] : (T)X[gk * xs + xo + gc];
+            rb[r] = v;
         }
     };
     auto swrite = [&](int s) {

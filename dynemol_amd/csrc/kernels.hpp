@@ -81,6 +81,8 @@ struct MatvecArgs {
     int x_stride, x_off;    // column selection: uses x[i*x_stride + x_off + r], r < R
     int out_stride, out_off;
     const float* M32;       // FP32 copy of M (mixed and FP32-storage precision modes; may be null)
+    const float* x32;       // mixed mode: FP32 copy of x (same layout), read by the FP32 GEMM; may be null
+    float* out32;           // mixed mode: the epilogue also writes T_k here in FP32 (same layout); may be null
 };
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
