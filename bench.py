@@ -119,7 +119,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     }
 
 
-SYM_VARIANTS = (16, 17, 18)  # symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4)
+SYM_VARIANTS = (16, 17, 18, 19, 20)  # symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4)
 
 
 def term_bytes(N, R):

@@ -1449,6 +1449,154 @@ static void launch_symm(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
 }
 
+// ------------------------------------------------------------------ symmetric, column strips
+// Variant 20: the segments and partial layout of variants 16-18 (so
+// sym_reduce_kernel is its phase 2), with no LDS staging of the tile and no
+// barrier per tile.  Wave w owns columns 16w..16w+15 of every tile of the
+// segment (all 64 rows), in 16 registers (rg, ks): lane 16k + 4b + i holds
+// H[16rg + 4b + i][16w + 4ks + k] (4 columns x 128 B per load instruction).
+// FP64 MFMA 4x4x4 (4 blocks) lanes: A 16k + 4b + i, B 16k + 4b + j,
+// D 16i + 4b + j.
+//  * rows: each register is the A operand as loaded (block b = rows
+//    16rg + 4b..+3, reduction over the wave's columns 16w + 4ks + k); B is
+//    x_J (lane 16k + 4b + n: x_J[16w + 4ks + k][4h + n]).  The sums run over
+//    the whole segment; the four waves' row sums (each over its own columns)
+//    are added in LDS once per segment, in wave order;
+//  * columns: a product with the identity as B moves A[i][k] of lane
+//    16k + 4b + i to lane 16i + 4b + k, exactly (one product by 1, three by
+//    0), which is the A operand with the rows 16rg + 4b + k as the reduction
+//    index; B is x_I (lane 16k + 4b + n: x_I[16rg + 4b + k][4h + n]).  Summed
+//    over rg in the accumulator and over the four blocks (lanes 4 and 8
+//    apart) by two DPP row rotations, these are the complete column sums of
+//    the wave's 16 columns, written straight to the tile's partial.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
+
+template <int R>
+__global__ __launch_bounds__(256) void symm3_tiles_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
+                                                          double* __restrict__ part, int ld, int N, int x_stride,
+                                                          int x_off, int S) {
+    constexpr int C2 = 2 * R;
+    constexpr int G = (C2 + 3) / 4;   // groups of 4 real columns
+    __shared__ double ys[4 * TILE * C2];
+    const int nb = ld / TILE;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sym_segment_start(mid, S) <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const int I = lo;
+    const int Ja = ((int)blockIdx.x - sym_segment_start(I, S)) * S, Jb = min(I + 1, Ja + S);
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i0 = I * TILE;
+    const int lk = lane >> 4, lb = (lane >> 2) & 3, lq = lane & 3;
+    const long xs2 = 2L * x_stride, xo2 = 2L * x_off;
+    // H's padding is [0 0; 0 I] and x rows >= N are read as row N - 1 (finite
+    // data): rows >= N of x never reach rows < N of the result.  Real columns
+    // >= 2R are read as column 2R - 1; their MFMA outputs are not stored.
+    auto xval = [&](int r, int h) { return xg[(long)min(r, N - 1) * xs2 + xo2 + min(4 * h + lq, C2 - 1)]; };
+    double xi[4][G];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int h = 0; h < G; ++h) xi[rg][h] = xval(i0 + 16 * rg + 4 * lb + lk, h);
+    const double ident = lk == lq ? 1.0 : 0.0;
+    double hn[16], xn[4][G];
+    auto fetch = [&](int J) {
+        const int j0 = J * TILE + 16 * w;
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                hn[4 * rg + ks] = Mg[(long)(j0 + 4 * ks + lk) * ld + i0 + 16 * rg + 4 * lb + lq];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int h = 0; h < G; ++h) xn[ks][h] = xval(j0 + 4 * ks + lk, h);
+    };
+    double yacc[4][G];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int h = 0; h < G; ++h) yacc[rg][h] = 0.0;
+    fetch(Ja);
+    for (int J = Ja; J < Jb; ++J) {
+        double hc[16], xj[4][G];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) hc[c] = hn[c];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int h = 0; h < G; ++h) xj[ks][h] = xn[ks][h];
+        if (J + 1 < Jb) fetch(J + 1);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+                for (int h = 0; h < G; ++h)
+                    yacc[rg][h] = __builtin_amdgcn_mfma_f64_4x4x4f64(hc[4 * rg + ks], xj[ks][h], yacc[rg][h], 0, 0, 0);
+        if (J == I) continue;  // diagonal tile: the row sums cover it
+        double zacc[4][G];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int h = 0; h < G; ++h) zacc[ks][h] = 0.0;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) {
+                const double tr = __builtin_amdgcn_mfma_f64_4x4x4f64(hc[4 * rg + ks], ident, 0.0, 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < G; ++h)
+                    zacc[ks][h] = __builtin_amdgcn_mfma_f64_4x4x4f64(tr, xi[rg][h], zacc[ks][h], 0, 0, 0);
+            }
+        // lane 16m + 4b + n, after the block sums: column 16w + 4ks + m, real
+        // column 4h + n for every b; lane b stores ks = b
+        const long tI = (long)I * (I - 1) / 2 + J;
+        double* pz = part + 2 * (tI * TILE * R + (long)(16 * w + 4 * lb + lk) * R);
+#pragma unroll
+        for (int h = 0; h < G; ++h) {
+            double v[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const double z = zacc[ks][h] + dpp_f64<DPP_ROW_ROR8>(zacc[ks][h]);
+                v[ks] = z + dpp_f64<DPP_ROW_ROR4>(z);
+            }
+            const double o = lb == 0 ? v[0] : lb == 1 ? v[1] : lb == 2 ? v[2] : v[3];
+            if (4 * h + lq < C2) pz[4 * h + lq] = o;
+        }
+    }
+    // row sums of wave w: lane 16m + 4b + n of (rg, h) is row 16rg + 4b + m, real column 4h + n
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int h = 0; h < G; ++h)
+            if (4 * h + lq < C2) ys[(w * TILE + 16 * rg + 4 * lb + lk) * C2 + 4 * h + lq] = yacc[rg][h];
+    __syncthreads();
+    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R);
+    for (int e = t; e < TILE * C2; e += 256)
+        py[e] = ((ys[e] + ys[TILE * C2 + e]) + ys[2 * TILE * C2 + e]) + ys[3 * TILE * C2 + e];
+}
+
+template <int R>
+static void launch_symm3(const MatvecArgs& a, hipStream_t st) {
+    const int nb = a.ld / TILE;
+    const int S = sym_seg_for(nb, 3);
+    hipLaunchKernelGGL((symm3_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
+                       reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
+                       a.x_stride, a.x_off, S);
+    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
+}
+
 // ------------------------------------------------------------------ symmetric, 2 x 2 super-tiles
 // Variant 19: the 4x4x4 matrix-core tile products of variant 18, with a
 // workgroup owning the super-tile (P, Q), Q <= P: block rows {2P, 2P+1} x
@@ -1632,8 +1780,9 @@ static void launch_symm2(const MatvecArgs& a, hipStream_t st) {
 // schedules; 5..8 = two-phase split-K; 15 = split-K matrix-core GEMM (full
 // operator); 16 = symmetric half storage, vector FMAs; 17 = symmetric half
 // storage, matrix-core tile products (16x16x4); 18 = the same with 4x4x4 products;
-// 19 = 4x4x4 products over 2 x 2 super-tiles.
-constexpr int MV_VARIANTS = 20;
+// 19 = 4x4x4 products over 2 x 2 super-tiles; 20 = 4x4x4 products over
+// per-wave column strips, without LDS staging of the tiles.
+constexpr int MV_VARIANTS = 21;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -1706,6 +1855,19 @@ void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
             case 2: launch_symm2<2>(a, st); break;
             case 4: launch_symm2<4>(a, st); break;
             default: launch_symm2<8>(a, st); break;
+        }
+        return;
+    }
+    if (variant == 20) {
+        if (!sym_supported(a.R, a.tri)) {
+            launch_matvec_variant(a, 1, st);
+            return;
+        }
+        switch (a.R) {
+            case 1: launch_symm3<1>(a, st); break;
+            case 2: launch_symm3<2>(a, st); break;
+            case 4: launch_symm3<4>(a, st); break;
+            default: launch_symm3<8>(a, st); break;
         }
         return;
     }
