@@ -108,6 +108,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     pops = [sysm.observables(w)[1] for w in range(2)]
     bounds = sysm.bounds()
     tuning = sysm.tuning()
+    ehrenfest = None if args.no_extra else time_ehrenfest(sysm)
     sysm.close()
     ctx.close()
     series_ms = st1["series_ms"] - st0["series_ms"]
@@ -115,8 +116,32 @@ def run_gpu(args, st, b, S, lumo, homo):
     return {
         "elapsed": elapsed, "rebuild_s": rebuild_s, "terms": terms, "series_ms": series_ms,
         "series_launches": launches, "pops": pops, "bounds": bounds.tolist(), "rebuilds": st1["rebuilds"] - st0["rebuilds"],
-        "tuning": tuning,
+        "tuning": tuning, "ehrenfest": ehrenfest,
     }
+
+
+def time_ehrenfest(sysm, reps=5):
+    """Diabatic-Ehrenfest kernel K = X o A - H' A on the drop-in's final state
+    (after the timed steps): device time per call and with K copied to the host,
+    and the invariant <K, S>_F = 0 as a check."""
+    from dynemol_amd import hip
+    N = sysm.N
+    t0 = time.perf_counter()
+    K = sysm.ehrenfest_kernel(0, 1)  # forms H' (one N^3 triangular product) first
+    first_s = time.perf_counter() - t0
+    S = sysm.matrix(hip.DH_MAT_S)
+    inv = float(abs(np.sum(K * S)) / np.sum(np.abs(K * S)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sysm.ehrenfest_kernel(0, 1, to_host=False)
+    dev_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    sysm.ehrenfest_kernel(0, 1)
+    host_s = time.perf_counter() - t0
+    log(f"ehrenfest kernel N={N}: {1e3 * dev_s:.2f} ms on the device, {1e3 * host_s:.2f} ms with K to the host")
+    return {"n_orbitals": N, "device_ms": round(1e3 * dev_s, 3), "with_host_copy_ms": round(1e3 * host_s, 3),
+            "first_call_ms": round(1e3 * first_s, 3), "gemm_tflops": round(2.0 * N ** 3 / dev_s / 1e12, 2),
+            "fp64_mfma_peak_tflops": 78.6, "K_S_frobenius_rel": inv}
 
 
 SYM_VARIANTS = (16, 17, 18, 19, 20)  # symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4)
@@ -474,6 +499,7 @@ def main():
         "chebyshev_terms_per_step": int(np.median(g["terms"])),
         "rebuild_ms": round(1e3 * g["rebuild_s"] / max(1, g["rebuilds"]), 3),
         "spectral_bounds_eV": [round(x, 4) for x in g["bounds"][:2]],
+        "ehrenfest": g["ehrenfest"],
         "batched": None,
         "serve": None,
     }

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdynemol_hip.so")
-SOURCES = ["api.hip", "overlap.hip", "dense.hip", "cheb.hip"]
+SOURCES = ["api.hip", "overlap.hip", "dense.hip", "cheb.hip", "ehrenfest.hip"]
 ARCH = os.environ.get("DH_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

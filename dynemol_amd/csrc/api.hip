@@ -34,6 +34,7 @@ struct dh_context {
     // shared workspace for operator builds (one build at a time per context)
     int ws_ld = 0;
     double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr;
+    double* Kd = nullptr;  // Ehrenfest kernel matrix (allocated on first use)
     int* d_info = nullptr;
     const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
     dh_system* dropin = nullptr;
@@ -65,6 +66,7 @@ struct dh_system {
     double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr, *norms1 = nullptr;
     // host state
     bool have_coords = false, ops_ready = false, phi_dirty = true, state_set = false, hprime_ready = false;
+    bool w_ready = false;  // the context's W holds L^-1 h of this system's current operators
     double bounds_h[6] = {0, 0, 0, 0, 0, 0};
     double emin = 0, emax = 0;
     int lanczos_steps = 48;
@@ -148,7 +150,8 @@ extern "C" int dh_init(int device, dh_context** out) {
 
 static void free_ws(dh_context* c) {
     hipFree(c->S); hipFree(c->h); hipFree(c->W); hipFree(c->Hp); hipFree(c->Dinv);
-    c->S = c->h = c->W = c->Hp = c->Dinv = nullptr;
+    if (c->Kd) hipFree(c->Kd);
+    c->S = c->h = c->W = c->Hp = c->Dinv = c->Kd = nullptr;
     c->ws_ld = 0;
     c->ws_owner = nullptr;
 }
@@ -534,6 +537,21 @@ static int autotune_system(dh_system* s) {
     return DH_OK;
 }
 
+// H' = S^-1 h = L^-T W into the context's Hp, from W = L^-1 h.  linv holds
+// L^-1 with a zero upper triangle: the product reads whole diagonal tiles.
+static void launch_hprime(dh_system* s, const double* linv) {
+    dh_context* c = s->ctx;
+    const int ld = s->ld;
+    Gemm g{};
+    g.M = ld; g.N = ld; g.K = ld;
+    g.A = linv; g.lda = ld; g.transA = 1;
+    g.B = c->W; g.ldb = ld;
+    g.C = c->Hp; g.ldc = ld;
+    g.alpha = 1.0; g.beta = 0.0; g.kmode = K_FROM_ROWTILE; g.batch = 1;
+    launch_gemm(g, c->stream);
+    s->hprime_ready = true;
+}
+
 extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
     CHECK_ARG(s, "dh_orthogonalize: null system");
     dh_context* c = s->ctx;
@@ -576,16 +594,8 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
         launch_symmetrize_from_lower(s->Hth, ld, st);
     }
     s->hprime_ready = false;
-    if (want_hprime) {  // H' = S^-1 h = L^-T W
-        Gemm g{};
-        g.M = ld; g.N = ld; g.K = ld;
-        g.A = s->Q2; g.lda = ld; g.transA = 1;
-        g.B = c->W; g.ldb = ld;
-        g.C = c->Hp; g.ldc = ld;
-        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_FROM_ROWTILE; g.batch = 1;
-        launch_gemm(g, st);
-        s->hprime_ready = true;
-    }
+    s->w_ready = true;
+    if (want_hprime) launch_hprime(s, s->Q2);  // Q2 is not mirrored yet
     launch_symmetrize_from_lower(s->Q1, ld, st);
     launch_symmetrize_from_lower(s->Q2, ld, st);
     DH_HIP(hipGetLastError());
@@ -1500,6 +1510,66 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
 
 // preprocess_ElHl_Chebyshev (ElHl_Chebyshev.f:110-129): Psi_ket = C_ket, Psi_bra = S C_bra,
 // i.e. phi_ket = L^T C_ket and phi_bra = L^-1 S C_bra = L^T C_bra, then the t = 0 outputs.
+// Diabatic-Ehrenfest kernel matrix from the resident state and H'
+// (diabatic-Ehren.f:100-116, :366-392; ehrenfestkernel2_gpu_, Taylor_gpu.cpp:800-875).
+// AO_bra = conj(S^-1 Psi_bra) and AO_ket = Psi_ket are the propagator's outputs
+// (ElHl_Chebyshev.f:277-279); A is built from them on the device, and the
+// context's W buffer (free once H' is formed) holds it.
+extern "C" int dh_ehrenfest_kernel(dh_system* s, int wp_el, int wp_hl, const double* X, int ldx, double* K, int ldk) {
+    CHECK_ARG(s, "dh_ehrenfest_kernel: null system");
+    CHECK_ARG(wp_el >= 0 && wp_el < s->W && wp_hl >= -1 && wp_hl < s->W,
+              "dh_ehrenfest_kernel: wavepacket index out of range");
+    CHECK_ARG((!K || ldk >= s->N) && (!X || ldx >= s->N), "dh_ehrenfest_kernel: leading dimension < N");
+    dh_context* c = s->ctx;
+    if (!s->ops_ready || c->ws_owner != s || c->ws_ld < s->ld || !(s->hprime_ready || s->w_ready)) {
+        dh::set_error("dh_ehrenfest_kernel: the operators of this system are not in the context workspace "
+                      "(build or load them again)");
+        return DH_ERR_STATE;
+    }
+    if (!s->state_set) {
+        dh::set_error("dh_ehrenfest_kernel: no wavepackets set");
+        return DH_ERR_STATE;
+    }
+    if (!X && s->n_atoms <= 0) {  // IP, k_WH and V_shift are filled only for systems with a basis
+        dh::set_error("dh_ehrenfest_kernel: X not given and the system has no basis to form it from");
+        return DH_ERR_STATE;
+    }
+    DH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int W = s->W, ld = s->ld;
+    if (!c->Kd) DALLOC(c->Kd, (size_t)c->ws_ld * c->ws_ld);
+    if (!s->hprime_ready) {  // W = L^-1 h is still there; Q2 is mirrored by now
+        launch_copy_lower(s->Q2, c->Kd, ld, st);
+        launch_hprime(s, c->Kd);
+    }
+    s->w_ready = false;  // W holds A from here on
+    if (s->phi_dirty) {  // states set in the AO basis: the outputs follow from phi
+        convert_ao_to_phi(s);
+        s->phi_dirty = false;
+        launch_outputs(s);
+    }
+    if (X) {  // the caller's X goes into K, which the elementwise pass overwrites in place with X o A
+        int rc = copy_matrix(c->Kd, ld, X, ldx, s->N, hipMemcpyHostToDevice, st);
+        if (rc) return rc;
+    }
+    EhrenfestArgs e{};
+    e.N = s->N; e.ld = ld;
+    e.bra = s->out; e.bra_stride = 3 * W; e.bra_el = W + wp_el; e.bra_hl = wp_hl >= 0 ? W + wp_hl : -1;
+    e.conj_bra = 1;
+    e.ket = s->out; e.ket_stride = 3 * W; e.ket_el = 2 * W + wp_el; e.ket_hl = wp_hl >= 0 ? 2 * W + wp_hl : -1;
+    e.X = X ? c->Kd : nullptr; e.ldx = ld;
+    e.ip = s->bd.ip; e.kwh = s->bd.kwh; e.vshift = s->bd.vshift;
+    e.A = c->W; e.K = c->Kd;
+    launch_ehrenfest(e, c->Hp, st);
+    DH_HIP(hipGetLastError());
+    if (K) {
+        int rc = copy_matrix(K, ldk, c->Kd, ld, s->N, hipMemcpyDeviceToHost, st);
+        if (rc) return rc;
+    }
+    DH_HIP(hipStreamSynchronize(st));
+    return DH_OK;
+}
+
 extern "C" int dh_set_initial_state(dh_system* s, int wp, const double* ao_bra, const double* ao_ket) {
     CHECK_ARG(s && ao_bra && ao_ket && wp >= 0 && wp < s->W, "dh_set_initial_state: bad argument");
     CHECK_ARG(s->ops_ready, "dh_set_initial_state: operators not built");

@@ -25,6 +25,17 @@ struct BasisDev {
     const int* atom_order;    // [n_atoms] atoms sorted by basis signature (groups equal elements)
 };
 
+// X_ij, i != j, of the extended-Hueckel Hamiltonian h_ij = X_ij S_ij
+// (hamiltonians.f:33-63); X_ii = IP_i + V_shift_i
+__host__ __device__ inline double huckel_x(double ipa, double ipb, double kwa, double kwb, double va, double vb) {
+    const double c1 = ipa - ipb, c2 = ipa + ipb;
+    const double c3 = (c1 / c2) * (c1 / c2);
+    const double c4 = (va + vb) * 0.5;
+    const double kwh = (kwa + kwb) * 0.5;
+    const double keff = kwh + c3 + c3 * c3 * (1.0 - kwh);
+    return keff * c2 * 0.5 + c4;
+}
+
 int launch_build_SH(const BasisDev& b, int N, int ld, double* S, double* h, double cutoff, double s_drop,
                     hipStream_t st);
 
@@ -48,6 +59,30 @@ struct Gemm {
     int c_acc;       // set by launch_gemm for alpha = -1, beta = 1: C starts in the accumulators
 };
 void launch_gemm(const Gemm& g, hipStream_t st);
+
+// ---------------------------------------------------------------- diabatic Ehrenfest
+// A = (rho + rho^T)/2 with rho_ij = Re(ket_j,el bra_i,el) - Re(ket_j,hl bra_i,hl),
+// then K = X o A - H' A.  bra and ket are row-major complex states (row r,
+// column c at r * stride + c); conj_bra conjugates bra on load; bra_hl < 0
+// leaves out the hole term.  X: ld x ld matrix (may alias K), or null to form
+// X_ij from ip, kwh and vshift.  A and K: ld x ld, written in full.
+struct EhrenfestArgs {
+    int N, ld;
+    const cplx* bra;
+    long bra_stride;
+    int bra_el, bra_hl, conj_bra;
+    const cplx* ket;
+    long ket_stride;
+    int ket_el, ket_hl;
+    const double* X;
+    long ldx;
+    const double *ip, *kwh, *vshift;
+    double* A;
+    double* K;
+};
+void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st);
+// dst (ld x ld) = lower triangle of src, zero above the diagonal
+void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st);
 
 // Cholesky S = L L^T in place (lower; upper zeroed) on an ld x ld padded
 // matrix, plus L^-1 in Linv.  info (device int) gets the first failing

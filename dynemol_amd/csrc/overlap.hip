@@ -403,14 +403,7 @@ __global__ void build_SH_kernel(const BasisDev b, double* S, double* h, int ld, 
                 s += solnorm[k] * (rl[RX(ma, -k, la)] * rl[RX(mb, -k, lb)] + rl[RX(ma, k, la)] * rl[RX(mb, k, lb)]);
             if (fabs(s) < s_drop) s = 0.0;
             const int a = a0 + ma + la, bo = b0 + mb + lb;
-            // X_ij (hamiltonians.f:33-63)
-            const double ipa = b.ip[a], ipb = b.ip[bo];
-            const double c1 = ipa - ipb, c2 = ipa + ipb;
-            const double c3 = (c1 / c2) * (c1 / c2);
-            const double c4 = (b.vshift[a] + b.vshift[bo]) * 0.5;
-            const double kwh = (b.kwh[a] + b.kwh[bo]) * 0.5;
-            const double keff = kwh + c3 + c3 * c3 * (1.0 - kwh);
-            const double hv = (keff * c2 * 0.5 + c4) * s;
+            const double hv = huckel_x(b.ip[a], b.ip[bo], b.kwh[a], b.kwh[bo], b.vshift[a], b.vshift[bo]) * s;
             S[(long)a + (long)ld * bo] = s;
             S[(long)bo + (long)ld * a] = s;
             h[(long)a + (long)ld * bo] = hv;

@@ -35,7 +35,7 @@ SYMBOLS = [
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
-    "dh_set_graph",
+    "dh_set_graph", "dh_ehrenfest_kernel",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read"]
@@ -85,6 +85,7 @@ def lib():
     L.dh_get_wavepacket.argtypes = [_vp, ctypes.c_int, _d, _d]
     L.dh_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
     L.dh_get_observables.argtypes = [_vp, ctypes.c_int, _d, _d, _d]
+    L.dh_ehrenfest_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int]
     L.dh_get_stats.argtypes = [_vp, _d]
     L.dh_propagation_elhl.argtypes = [_vp, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]
     L.dh_test_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int, _d,
@@ -261,6 +262,26 @@ class System:
         erg = np.zeros(2)
         _check(lib().dh_get_observables(self.h, wp, ao.ctypes.data_as(_d), _p(pops), _p(erg)), "dh_get_observables")
         return ao, pops, complex(erg[0], erg[1])
+
+    def ehrenfest_kernel(self, wp_el=0, wp_hl=1, X=None, to_host=True):
+        """Diabatic-Ehrenfest kernel K = X o A - H' A of the current state (N x N,
+        Fortran order); X = None forms the Hueckel factors from the system's basis.
+        to_host = False computes K on the device only and returns None."""
+        if not to_host:
+            xp = None if X is None else np.asfortranarray(X, dtype=np.float64)
+            _check(lib().dh_ehrenfest_kernel(self.h, wp_el, wp_hl, None if xp is None else xp.ctypes.data_as(_d),
+                                             self.N, None, self.N), "dh_ehrenfest_kernel")
+            return None
+        K = np.zeros((self.N, self.N), order="F")
+        if X is None:
+            xp, ldx = None, self.N
+        else:
+            X = np.asfortranarray(X, dtype=np.float64)
+            assert X.shape == (self.N, self.N)
+            xp, ldx = X.ctypes.data_as(_d), self.N
+        _check(lib().dh_ehrenfest_kernel(self.h, wp_el, wp_hl, xp, ldx, K.ctypes.data_as(_d), self.N),
+               "dh_ehrenfest_kernel")
+        return K
 
     def stats(self):
         out = np.zeros(9)

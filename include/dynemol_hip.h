@@ -117,6 +117,22 @@ int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
  * last (data_output.f:87-147); erg (2) = QuasiParticleEnergies
  * sum_ij conj(ao_bra)_i h_ij Psi_ket_j (ElHl_Chebyshev.f:329-371). */
 int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, double *erg);
+/* Diabatic-Ehrenfest kernel matrix of the current state, on the device:
+ * K = X o A - H' A (N x N, column-major, ldk >= N, written to host K), with
+ * A = (rho + rho^T) / 2 and rho_ij = Re(AO_ket_j,el AO_bra_i,el)
+ * - Re(AO_ket_j,hl AO_bra_i,hl), where AO_bra = conj(S^-1 Psi_bra) and
+ * AO_ket = Psi_ket of wavepackets wp_el and wp_hl (wp_hl = -1: no hole term).
+ * X (N x N, ldx >= N) holds the extended-Hueckel factors h_ij = X_ij S_ij, or is
+ * NULL to form them from the system's basis (systems built from coordinates).
+ * H' is formed on the device on the first call after each operator build
+ * (from L^-1 h, which the build leaves in the context) and stays there.  K may be
+ * NULL: the matrix is then only computed (it stays in the context until the next
+ * call).  A later build or load of another system on the same context takes
+ * the workspace: the call then fails with DH_ERR_STATE until this system is
+ * built again.  Replaces ehrenfestkernel2_gpu_ (Taylor_gpu.cpp:800-875) with the
+ * calculate_rho / Huckel_stuff steps before it (diabatic-Ehren.f:100-116,
+ * 250-276, 366-392); the host copies of H', A and X it needed are gone. */
+int dh_ehrenfest_kernel(dh_system *sys, int wp_el, int wp_hl, const double *X, int ldx, double *K, int ldk);
 /* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
  * retries, last norm drift, term-series GPU ms (HIP events, when timing is
  * on), term launches timed, operator rebuilds, reserved]. */

@@ -21,6 +21,9 @@
  *                         :224-239 (coefficient), :290-303 (isConverged)
  *   orc_populations       data_output.f:87-147,242-263 (Populations_mtx, pop_Slater)
  *   orc_qp_energy         ElHl_Chebyshev.f:329-371 (QuasiParticleEnergies)
+ *   orc_huckel_x          hamiltonians.f:33-63, diabatic-Ehren.f:250-276 (Huckel_stuff: X_ij)
+ *   orc_ehrenfest_kernel  diabatic-Ehren.f:100-116 (A = (rho + rho^T)/2, K = X o A - H' A),
+ *                         :366-392 (calculate_rho)
  *
  * Matrices are column-major (Fortran order), leading dimension N.
  * Complex vectors are interleaved (re, im) double pairs.
@@ -29,7 +32,9 @@
  * 1s-1s closed form; ROTAR against an independently built real-harmonic
  * rotation; the assembled S against invariants (symmetry, unit diagonal,
  * rotation/translation invariance of its spectrum); the Taylor propagator
- * against the exact generalised-eigen propagator (scipy.linalg.eigh).
+ * against the exact generalised-eigen propagator (scipy.linalg.eigh); X_ij
+ * against h_ij / S_ij; the Ehrenfest kernel against <K, S>_F = tr(hA) - tr(hA) = 0,
+ * the symmetry of A and u^T A u = |u^T psi_e|^2 - |u^T psi_h|^2 for real u.
  */
 #include <complex.h>
 #include <math.h>
@@ -503,23 +508,24 @@ int orc_overlap(int n_atoms, const double *coords, const int *first_orb, const i
 }
 
 /* X_ij * S_ij: hamiltonians.f:33-63 and ElHl_Chebyshev.f:296-323 */
+/* X_ij of hamiltonians.f:33-63 */
+static double huckel_x_ij(long i, long j, const double *ip, const double *kwh, const double *vshift) {
+    if (i == j) return ip[i] + vshift[i];
+    double c1 = ip[i] - ip[j];
+    double c2 = ip[i] + ip[j];
+    double c3 = (c1 / c2) * (c1 / c2);
+    double c4 = (vshift[i] + vshift[j]) * 0.5;
+    double k = (kwh[i] + kwh[j]) * 0.5;
+    double keff = k + c3 + c3 * c3 * (1.0 - k);
+    return keff * c2 * 0.5 + c4;
+}
+
 void orc_huckel(int n_orb, const double *ip, const double *kwh, const double *vshift, const double *S, double *h) {
     const long N = n_orb;
 #pragma omp parallel for schedule(static)
     for (long j = 0; j < N; ++j) {
         for (long i = 0; i <= j; ++i) {
-            double x;
-            if (i == j) {
-                x = ip[i] + vshift[i];
-            } else {
-                double c1 = ip[i] - ip[j];
-                double c2 = ip[i] + ip[j];
-                double c3 = (c1 / c2) * (c1 / c2);
-                double c4 = (vshift[i] + vshift[j]) * 0.5;
-                double k = (kwh[i] + kwh[j]) * 0.5;
-                double keff = k + c3 + c3 * c3 * (1.0 - k);
-                x = keff * c2 * 0.5 + c4;
-            }
+            const double x = huckel_x_ij(i, j, ip, kwh, vshift);
             h[i + N * j] = x * S[i + N * j];
             h[j + N * i] = h[i + N * j];
         }
@@ -848,4 +854,53 @@ void orc_qp_energy(int n, const double *ao_bra, const double *H, const double *a
     }
     erg[0] = er;
     erg[1] = ei;
+}
+
+/* Huckel_stuff (diabatic-Ehren.f:250-276): the full symmetric X_ij matrix. */
+void orc_huckel_x(int n_orb, const double *ip, const double *kwh, const double *vshift, double *X) {
+    const long N = n_orb;
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < N; ++j)
+        for (long i = 0; i <= j; ++i) {
+            X[i + N * j] = huckel_x_ij(i, j, ip, kwh, vshift);
+            X[j + N * i] = X[i + N * j];
+        }
+}
+
+/*
+ * Diabatic-Ehrenfest kernel matrix, CPU branch of Diabatic_Ehrenfest
+ * (diabatic-Ehren.f:100-116):
+ *   rho_ij = Re(ket_j,1 bra_i,1) - Re(ket_j,2 bra_i,2)   calculate_rho (:366-392)
+ *   A      = (rho + rho^T) / two                         (:104)
+ *   K      = X * A - matmul(H', A)                       (:112-116)
+ * bra, ket: AO_bra(N, n_part), AO_ket(N, n_part), complex column-major, bra as
+ * the caller holds it (conjugated, ElHl_Chebyshev.f:277-279); n_part = 1 drops
+ * the hole term.  A (may be NULL) receives A.
+ */
+void orc_ehrenfest_kernel(int n, int n_part, const double *Hp, const double *X, const double *bra,
+                          const double *ket, double *A_out, double *K) {
+    const long N = n;
+    double *rho = malloc(sizeof(double) * N * N);
+    double *A = malloc(sizeof(double) * N * N);
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < N; ++j)
+        for (long i = 0; i < N; ++i) {
+            double r = 0.0;
+            for (int p = 0; p < n_part; ++p) {
+                const double *kj = &ket[2 * (j + N * p)], *bi = &bra[2 * (i + N * p)];
+                const double v = kj[0] * bi[0] - kj[1] * bi[1];
+                r = p == 0 ? v : r - v;
+            }
+            rho[i + N * j] = r;
+        }
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < N; ++j)
+        for (long i = 0; i < N; ++i) A[i + N * j] = (rho[i + N * j] + rho[j + N * i]) / 2.0;
+    orc_dgemm_nn(n, Hp, A, K);  /* B = H' A */
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < N; ++j)
+        for (long i = 0; i < N; ++i) K[i + N * j] = X[i + N * j] * A[i + N * j] - K[i + N * j];
+    if (A_out) memcpy(A_out, A, sizeof(double) * N * N);
+    free(rho);
+    free(A);
 }

@@ -59,6 +59,8 @@ def lib():
         L.orc_bra_x_op.argtypes = [ctypes.c_int, _d, _d, _d]
         L.orc_populations.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d]
         L.orc_qp_energy.argtypes = [ctypes.c_int, _d, _d, _d, _d]
+        L.orc_huckel_x.argtypes = [ctypes.c_int, _d, _d, _d, _d]
+        L.orc_ehrenfest_kernel.argtypes = [ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _d, _d]
         L.orc_util_init()
         _LIB = L
     return _LIB
@@ -124,6 +126,31 @@ def huckel(basis, S):
     lib().orc_huckel(N, _p(basis.ip), _p(basis.k_wh), _p(basis.v_shift), S.ctypes.data_as(_d),
                      h.ctypes.data_as(_d))
     return h
+
+
+def huckel_x(basis):
+    """X_ij with h_ij = X_ij S_ij (hamiltonians.f:33-63; Huckel_stuff, diabatic-Ehren.f:250-276)."""
+    N = basis.N
+    X = np.zeros((N, N), order="F")
+    lib().orc_huckel_x(N, _p(basis.ip), _p(basis.k_wh), _p(basis.v_shift), X.ctypes.data_as(_d))
+    return X
+
+
+def ehrenfest_kernel(Hp, X, ao_bra, ao_ket):
+    """K = X o A - H' A and A = (rho + rho^T)/2 (diabatic-Ehren.f:100-116, 366-392).
+    ao_bra, ao_ket: N x n_part complex (electron column 0, hole column 1), ao_bra
+    already conjugated as the reference passes it.  Returns (K, A)."""
+    N = Hp.shape[0]
+    b = np.asfortranarray(np.asarray(ao_bra, dtype=np.complex128).reshape(N, -1))
+    k = np.asfortranarray(np.asarray(ao_ket, dtype=np.complex128).reshape(N, -1))
+    assert b.shape == k.shape and b.shape[1] in (1, 2)
+    Hp = np.asfortranarray(Hp, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    K = np.zeros((N, N), order="F")
+    A = np.zeros((N, N), order="F")
+    lib().orc_ehrenfest_kernel(N, b.shape[1], Hp.ctypes.data_as(_d), X.ctypes.data_as(_d), b.ctypes.data_as(_d),
+                               k.ctypes.data_as(_d), A.ctypes.data_as(_d), K.ctypes.data_as(_d))
+    return K, A
 
 
 def cholesky(S):

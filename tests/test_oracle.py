@@ -289,3 +289,60 @@ def test_populations_and_energy_invariants(c1):
     assert abs(pops[2] - np.real(np.conj(ket) @ S @ ket)) < 1e-9
     e = O.qp_energy(np.conj(np.linalg.solve(S, bra)), h, ket)
     assert abs(e - np.conj(ket) @ h @ ket) < 1e-8 * abs(e)
+
+
+# ---------------------------------------------------------------- diabatic-Ehrenfest kernel
+def test_huckel_x_is_h_over_S(c1):
+    st, b, S, h = c1
+    X = O.huckel_x(b)
+    assert np.array_equal(X, X.T)
+    assert np.array_equal(np.diag(X), b.ip + b.v_shift)
+    assert np.array_equal(X * S, h)  # h_ij = X_ij S_ij, the same product
+
+
+def _ehrenfest_case(c1, seed=7):
+    st, b, S, h = c1
+    rng = np.random.default_rng(seed)
+    psi = rng.normal(size=(b.N, 2)) + 1j * rng.normal(size=(b.N, 2))
+    Hp = np.linalg.solve(S, h)
+    # pure states: Psi_bra = S psi, so AO_bra = conj(S^-1 Psi_bra) = conj(psi), AO_ket = psi
+    K, A = O.ehrenfest_kernel(Hp, O.huckel_x(b), np.conj(psi), psi)
+    return S, h, Hp, psi, K, A
+
+
+def _frobenius_with_S_is_zero(K, S, scale):
+    return abs(np.sum(K * S)) <= 1e-12 * scale
+
+
+def test_ehrenfest_kernel_invariants(c1):
+    """<K, S>_F = sum_ij (X_ij A_ij - (H'A)_ij) S_ij = tr(hA) - tr(S S^-1 h A) = 0 for
+    symmetric A; u^T A u = |u^T psi_e|^2 - |u^T psi_h|^2 for real u (rho = Re(conj(psi) psi^T))."""
+    S, h, Hp, psi, K, A = _ehrenfest_case(c1)
+    assert np.array_equal(A, A.T)
+    scale = np.sum(np.abs(h * A))
+    assert _frobenius_with_S_is_zero(K, S, scale)
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        u = rng.normal(size=S.shape[0])
+        ref = abs(u @ psi[:, 0]) ** 2 - abs(u @ psi[:, 1]) ** 2
+        assert abs(u @ A @ u - ref) <= 1e-12 * (abs(u @ psi[:, 0]) ** 2 + abs(u @ psi[:, 1]) ** 2)
+    # the electron alone
+    K1, A1 = O.ehrenfest_kernel(Hp, O.huckel_x(c1[1]), np.conj(psi[:, :1]), psi[:, :1])
+    u = rng.normal(size=S.shape[0])
+    assert abs(u @ A1 @ u - abs(u @ psi[:, 0]) ** 2) <= 1e-12 * abs(u @ psi[:, 0]) ** 2
+    assert _frobenius_with_S_is_zero(K1, S, np.sum(np.abs(h * A1)))
+
+
+def test_ehrenfest_invariants_catch_planted_mistakes(c1):
+    """Each invariant above fails for a plausible slip in the formula."""
+    S, h, Hp, psi, K, A = _ehrenfest_case(c1)
+    X = O.huckel_x(c1[1])
+    scale = np.sum(np.abs(h * A))
+    assert not _frobenius_with_S_is_zero(X * A + Hp @ A, S, scale)    # wrong sign of H'A
+    assert not _frobenius_with_S_is_zero(X * A - Hp.T @ A, S, scale)  # transposed H'
+    assert not _frobenius_with_S_is_zero(X * A - A @ Hp, S, scale)    # product order swapped
+    # rho without the conjugate of the bra: Re(psi_i psi_j)
+    r = np.real(np.outer(psi[:, 0], psi[:, 0])) - np.real(np.outer(psi[:, 1], psi[:, 1]))
+    u = np.random.default_rng(9).normal(size=S.shape[0])
+    ref = abs(u @ psi[:, 0]) ** 2 - abs(u @ psi[:, 1]) ** 2
+    assert abs(u @ r @ u - ref) > 1e-6 * abs(ref)
