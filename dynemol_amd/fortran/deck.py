@@ -41,4 +41,6 @@ def read_output(path, N, n_frag, n_steps):
     ket = psi[:, :, 0] + 1j * psi[:, :, 1]
     bra = psi[:, :, 2] + 1j * psi[:, :, 3]
     erg = rows[nd + 2 * N]
-    return dyn, bra, ket, np.array([erg[0] + 1j * erg[1], erg[2] + 1j * erg[3]])
+    kern = rows[nd + 2 * N + 1:nd + 3 * N + 1]  # Ehrenfest kernel, column i on line i
+    K = np.stack(kern).T if len(kern) == N else None
+    return dyn, bra, ket, np.array([erg[0] + 1j * erg[1], erg[2] + 1j * erg[3]]), K

@@ -95,6 +95,14 @@ module dynemol_hip
             complex(c_double_complex) :: ao_bra(*)
             real(c_double) :: pops(*), erg(2)
         end function
+        ! X: c_loc of real*8 X(ldx, N), or c_null_ptr to form X_ij from the system's basis
+        integer(c_int) function dh_ehrenfest_kernel(sys, wp_el, wp_hl, X, ldx, K, ldk) bind(C, name="dh_ehrenfest_kernel")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: sys
+            integer(c_int), value :: wp_el, wp_hl, ldx, ldk
+            type(c_ptr), value :: X
+            real(c_double) :: K(*)
+        end function
         integer(c_int) function dh_propagation_elhl(ctx, N, S, h, H_out, AO_bra, AO_ket, PSI_bra, PSI_ket, &
                 t_init, t_max, tau, save_tau) bind(C, name="dh_propagation_elhl")
             import :: c_int, c_double, c_double_complex, c_ptr

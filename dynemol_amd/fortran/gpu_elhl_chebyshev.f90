@@ -31,7 +31,7 @@ module gpu_ElHl_Chebyshev_m
     use dh_types_m
     implicit none
     private
-    public :: gpu_preprocess_ElHl_Chebyshev, gpu_ElHl_Chebyshev, gpu_ElHl_finalize, QM_wp_erg
+    public :: gpu_preprocess_ElHl_Chebyshev, gpu_ElHl_Chebyshev, gpu_ElHl_finalize, QM_wp_erg, gpu_EhrenfestKernel
 
     real*8, parameter :: frame_step = 1.d0      ! parameters_m frame_step for slice_Cheb
     type(c_ptr), save :: ctx = c_null_ptr, sysh = c_null_ptr
@@ -154,6 +154,17 @@ contains
         end do
         first_call = .false.
     end subroutine gpu_ElHl_Chebyshev
+
+    ! Diabatic-Ehrenfest kernel matrix of the current electron/hole state, for
+    ! Diabatic_Ehrenfest (diabatic-Ehren.f:100-116): replaces calculate_rho,
+    ! A_ad_nd = (rho_eh + transpose(rho_eh))/two, Huckel_stuff and
+    ! EhrenfestKernel_gpu(N, H_prime, A_ad_nd, X_ij, Kernel).  H', A and X stay on
+    ! the device; only Kernel(N, N) comes back.
+    subroutine gpu_EhrenfestKernel(Kernel)
+        real*8, intent(out) :: Kernel(:, :)
+        call check(dh_ehrenfest_kernel(sysh, 0_c_int, 1_c_int, c_null_ptr, int(size(Kernel, 1), c_int), Kernel, &
+                                       int(size(Kernel, 1), c_int)), 'dh_ehrenfest_kernel')
+    end subroutine
 
     subroutine gpu_ElHl_finalize()
         integer(c_int) :: rc

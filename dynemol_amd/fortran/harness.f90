@@ -9,6 +9,7 @@ program dh_harness
     type(STO_basis), allocatable :: basis(:)
     type(f_time) :: QDyn
     complex*16, allocatable :: AO_bra(:, :), AO_ket(:, :), Dual_bra(:, :), Dual_ket(:, :)
+    real*8, allocatable :: Kernel(:, :)
     integer :: N, na, nf, nsteps, i, a, it, p, rebuild_every
     real*8 :: delta_t, t, re, im, jitter(3)
     character(len=512) :: fin, fout
@@ -64,6 +65,11 @@ program dh_harness
         end do
     end do
     write(11, '(4es24.16)') real(QM_wp_erg(1)), aimag(QM_wp_erg(1)), real(QM_wp_erg(2)), aimag(QM_wp_erg(2))
+    allocate(Kernel(N, N))  ! diabatic-Ehrenfest kernel of the final state, one column per line
+    call gpu_EhrenfestKernel(Kernel)
+    do i = 1, N
+        write(11, '(*(es24.16))') Kernel(:, i)
+    end do
     close(11)
     call gpu_ElHl_finalize()
 end program dh_harness

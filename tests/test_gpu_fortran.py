@@ -32,7 +32,7 @@ def test_fortran_harness_matches_oracle(tmp_path, rebuild_every):
     write_deck(str(deck), st, b, C[:, 208], C[:, 207], nsteps, DT, rebuild_every, disp)
     r = subprocess.run([HARNESS, str(deck), str(out)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    dyn, bra, ket, erg = read_output(str(out), b.N, 2, nsteps)
+    dyn, bra, ket, erg, K = read_output(str(out), b.N, 2, nsteps)
 
     ref = O.ElHlReference(b, 2, np.stack([S @ C[:, 208], S @ C[:, 207]], 1), np.stack([C[:, 208], C[:, 207]], 1) + 0j,
                           err_tol=1e-13, norm_tol=1e-11)
@@ -51,3 +51,9 @@ def test_fortran_harness_matches_oracle(tmp_path, rebuild_every):
         assert np.abs(ket[p] - ref.psi_ket[:, p]).max() <= 1e-9 * np.abs(ket[p]).max()
         assert np.abs(bra[p] - ref.psi_bra[:, p]).max() <= 1e-9 * np.abs(bra[p]).max()
         assert abs(erg[p] - o["erg"][p]) < 1e-8
+    # diabatic-Ehrenfest kernel of the final state through gpu_EhrenfestKernel: the
+    # oracle's from the oracle's own final state (Psi to 1e-9, so K to 1e-8)
+    Sf = O.overlap(coords, b)
+    Hp = O.dgemm(O.syinvert(Sf), O.huckel(b, Sf))
+    Kref, _ = O.ehrenfest_kernel(Hp, O.huckel_x(b), np.conj(np.linalg.solve(Sf, ref.psi_bra)), ref.psi_ket)
+    assert K is not None and np.abs(K - Kref).max() <= 1e-8 * np.abs(Kref).max()
