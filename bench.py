@@ -108,7 +108,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     pops = [sysm.observables(w)[1] for w in range(2)]
     bounds = sysm.bounds()
     tuning = sysm.tuning()
-    ehrenfest = None if args.no_extra else time_ehrenfest(sysm)
+    ehrenfest = None if args.no_ehrenfest else time_ehrenfest(sysm)
     sysm.close()
     ctx.close()
     series_ms = st1["series_ms"] - st0["series_ms"]
@@ -404,6 +404,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ehrenfest", action="store_true",
+                    help="skip the diabatic-Ehrenfest kernel timing after the drop-in steps")
     ap.add_argument("--batched-wp", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serve-requests", type=int, default=10)

@@ -16,37 +16,59 @@ namespace dh {
 // Re(a b) written as in diabatic-Ehren.f:364
 __device__ inline double re_mul(cplx a, cplx b) { return a.re * b.re - a.im * b.im; }
 
-// Thread (i, j): i along the 256 threads of a block (coalesced column
-// segments of A and K), j = blockIdx.y.  Rows and columns >= N of A and K are
-// written as 0.
+// Thread i of a workgroup owns row i for EH_JB consecutive columns j: its
+// row-side bra and ket values (and IP, k_WH, V_shift) are loaded once, the
+// column-side ones are the same for the whole workgroup.  A and K are written
+// as coalesced column segments; rows and columns >= N are written as 0.
+constexpr int EH_JB = 32;
+
 __global__ __launch_bounds__(256) void ehrenfest_prep_kernel(EhrenfestArgs e) {
-    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= e.ld) return;
-    const long o = (long)j * e.ld + i;
-    if (i >= e.N || j >= e.N) {
-        e.A[o] = 0.0;
-        e.K[o] = 0.0;
-        return;
-    }
+    const int j0 = blockIdx.y * EH_JB, j1 = min(e.ld, j0 + EH_JB);
     auto bra = [&](int r, int col) {
         cplx v = e.bra[(long)r * e.bra_stride + col];
         if (e.conj_bra) v.im = -v.im;
         return v;
     };
     auto ket = [&](int r, int col) { return e.ket[(long)r * e.ket_stride + col]; };
-    double rij = re_mul(ket(j, e.ket_el), bra(i, e.bra_el));
-    double rji = re_mul(ket(i, e.ket_el), bra(j, e.bra_el));
-    if (e.bra_hl >= 0) {
-        rij -= re_mul(ket(j, e.ket_hl), bra(i, e.bra_hl));
-        rji -= re_mul(ket(i, e.ket_hl), bra(j, e.bra_hl));
+    const bool row = i < e.N, hole = e.bra_hl >= 0;
+    cplx bie{0.0, 0.0}, bih{0.0, 0.0}, kie{0.0, 0.0}, kih{0.0, 0.0};
+    double ipi = 0.0, kwi = 0.0, vsi = 0.0;
+    if (row) {
+        bie = bra(i, e.bra_el);
+        kie = ket(i, e.ket_el);
+        if (hole) {
+            bih = bra(i, e.bra_hl);
+            kih = ket(i, e.ket_hl);
+        }
+        if (!e.X) {
+            ipi = e.ip[i];
+            kwi = e.kwh[i];
+            vsi = e.vshift[i];
+        }
     }
-    const double a = (rij + rji) * 0.5;
-    double x;
-    if (e.X) x = e.X[(long)j * e.ldx + i];
-    else if (i == j) x = e.ip[i] + e.vshift[i];
-    else x = huckel_x(e.ip[i], e.ip[j], e.kwh[i], e.kwh[j], e.vshift[i], e.vshift[j]);
-    e.A[o] = a;
-    e.K[o] = x * a;
+    for (int j = j0; j < j1; ++j) {
+        const long o = (long)j * e.ld + i;
+        if (!row || j >= e.N) {
+            e.A[o] = 0.0;
+            e.K[o] = 0.0;
+            continue;
+        }
+        double rij = re_mul(ket(j, e.ket_el), bie);
+        double rji = re_mul(kie, bra(j, e.bra_el));
+        if (hole) {
+            rij -= re_mul(ket(j, e.ket_hl), bih);
+            rji -= re_mul(kih, bra(j, e.bra_hl));
+        }
+        const double a = (rij + rji) * 0.5;
+        double x;
+        if (e.X) x = e.X[(long)j * e.ldx + i];
+        else if (i == j) x = ipi + vsi;
+        else x = huckel_x(ipi, e.ip[j], kwi, e.kwh[j], vsi, e.vshift[j]);
+        e.A[o] = a;
+        e.K[o] = x * a;
+    }
 }
 
 // dst = the lower triangle of src (diagonal included), zero above
@@ -63,7 +85,8 @@ void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st) {
 }
 
 void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st) {
-    hipLaunchKernelGGL(ehrenfest_prep_kernel, dim3((e.ld + 255) / 256, e.ld), dim3(256), 0, st, e);
+    hipLaunchKernelGGL(ehrenfest_prep_kernel, dim3((e.ld + 255) / 256, (e.ld + EH_JB - 1) / EH_JB), dim3(256), 0, st,
+                       e);
     Gemm g{};
     g.M = e.N; g.N = e.N; g.K = e.N;
     g.A = Hp; g.lda = e.ld;
