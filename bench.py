@@ -54,13 +54,24 @@ def device_S_h(st, b):
     return S, h
 
 
+def homo_index(st):
+    """0-based HOMO index of the (h, S) pencil: valence electrons / 2 - 1."""
+    nel = sum({"Ti": 4, "O": 6, "C": 4, "H": 1, "N": 5}[s] for s in st.symbols)
+    return nel // 2 - 1
+
+
+HOST_EIGH_S = None  # host time of the setup's LUMO/HOMO solve, reported beside the device one
+
+
 def initial_states(st, b):
     """LUMO/HOMO of the (h, S) pencil (whole system flagged El and Hl, FMO.f:294-350)."""
+    global HOST_EIGH_S
     import scipy.linalg as sl
     S, h = device_S_h(st, b)
-    nel = sum({"Ti": 4, "O": 6, "C": 4, "H": 1, "N": 5}[s] for s in st.symbols)
-    homo = nel // 2 - 1
+    homo = homo_index(st)
+    t0 = time.perf_counter()
     E, C = sl.eigh(h, S, subset_by_index=[homo, homo + 1])
+    HOST_EIGH_S = time.perf_counter() - t0
     return S, h, C[:, 1], C[:, 0], E
 
 
@@ -109,6 +120,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     bounds = sysm.bounds()
     tuning = sysm.tuning()
     ehrenfest = None if args.no_ehrenfest else time_ehrenfest(sysm)
+    eigen = None if args.no_ehrenfest else time_eigenpairs(sysm, homo_index(st))
     sysm.close()
     ctx.close()
     series_ms = st1["series_ms"] - st0["series_ms"]
@@ -116,8 +128,24 @@ def run_gpu(args, st, b, S, lumo, homo):
     return {
         "elapsed": elapsed, "rebuild_s": rebuild_s, "terms": terms, "series_ms": series_ms,
         "series_launches": launches, "pops": pops, "bounds": bounds.tolist(), "rebuilds": st1["rebuilds"] - st0["rebuilds"],
-        "tuning": tuning, "ehrenfest": ehrenfest,
+        "tuning": tuning, "ehrenfest": ehrenfest, "eigenpairs": eigen,
     }
+
+
+def time_eigenpairs(sysm, homo, reps=3):
+    """HOMO and LUMO of the final geometry's (h, S) on the device (dh_eigenpairs),
+    beside the host scipy.linalg.eigh subset solve of the setup."""
+    k = homo + 1  # 1-based HOMO
+    sysm.eigenpairs(k, k + 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        E, C = sysm.eigenpairs(k, k + 1)
+    dev_s = (time.perf_counter() - t0) / reps
+    log(f"eigenpairs HOMO/LUMO N={sysm.N}: {1e3 * dev_s:.1f} ms on the device"
+        + (f", setup's host eigh {1e3 * HOST_EIGH_S:.0f} ms" if HOST_EIGH_S else ""))
+    return {"n_orbitals": sysm.N, "indices_1based": [k, k + 1], "device_ms": round(1e3 * dev_s, 2),
+            "energies_eV": [round(float(x), 6) for x in E],
+            "host_scipy_eigh_ms": round(1e3 * HOST_EIGH_S, 1) if HOST_EIGH_S else None}
 
 
 def time_ehrenfest(sysm, reps=5):
@@ -502,6 +530,7 @@ def main():
         "rebuild_ms": round(1e3 * g["rebuild_s"] / max(1, g["rebuilds"]), 3),
         "spectral_bounds_eV": [round(x, 4) for x in g["bounds"][:2]],
         "ehrenfest": g["ehrenfest"],
+        "eigenpairs": g["eigenpairs"],
         "batched": None,
         "serve": None,
     }

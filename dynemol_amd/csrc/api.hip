@@ -34,7 +34,8 @@ struct dh_context {
     // shared workspace for operator builds (one build at a time per context)
     int ws_ld = 0;
     double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr;
-    double* Kd = nullptr;  // Ehrenfest kernel matrix (allocated on first use)
+    double* Kd = nullptr;  // Ehrenfest kernel matrix / eigensolver scratch (allocated on first use)
+    EigenSolver* eig = nullptr;  // dense eigensolver handle (created on first use)
     int* d_info = nullptr;
     const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
     dh_system* dropin = nullptr;
@@ -197,6 +198,7 @@ extern "C" int dh_finalize(dh_context* c) {
     if (c->dropin) dh_system_destroy(c->dropin);
     hipStreamSynchronize(c->stream);
     free_ws(c);
+    eigen_destroy(c->eig);
     hipFree(c->d_info);
     hipStreamDestroy(c->stream);
     delete c;
@@ -651,6 +653,15 @@ static int get_column(dh_system* s, double* host, const cplx* src, long src_stri
     return DH_OK;
 }
 
+// rows x cols column-major block between pitches ldd and lds
+static int copy_block(double* dst, long ldd, const double* src, long lds, int rows, int cols, hipMemcpyKind kind,
+                      hipStream_t st) {
+    DH_HIP(hipMemcpy2DAsync(dst, sizeof(double) * ldd, src, sizeof(double) * lds, sizeof(double) * rows, cols, kind,
+                            st));
+    return DH_OK;
+}
+
+// n x n matrix
 static int copy_matrix(double* dst, long ldd, const double* src, long lds, int n, hipMemcpyKind kind,
                        hipStream_t st) {
     if (ldd == lds) {
@@ -1568,6 +1579,61 @@ extern "C" int dh_ehrenfest_kernel(dh_system* s, int wp_el, int wp_hl, const dou
     }
     DH_HIP(hipStreamSynchronize(st));
     return DH_OK;
+}
+
+// Eigenpairs il..iu (1-based, ascending) of the pencil (h, S) of the system's
+// current operators: the generalised eigenproblem FMO solves for the initial
+// electron and hole orbitals (FMO.f:294-350; xPU_dsygvd, GPU_Interface.cpp:705-853).
+extern "C" int dh_eigenpairs(dh_system* s, int il, int iu, double* E, double* C, int ldc) {
+    CHECK_ARG(s && il >= 1 && il <= iu && iu <= s->N, "dh_eigenpairs: need 1 <= il <= iu <= N");
+    CHECK_ARG(!C || ldc >= s->N, "dh_eigenpairs: ldc < N");
+    if (!s->ops_ready) {
+        dh::set_error("dh_eigenpairs: operators not built");
+        return DH_ERR_STATE;
+    }
+    dh_context* c = s->ctx;
+    DH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int ld = s->ld, N = s->N, nev = iu - il + 1;
+    int rc = ensure_ws(c, ld);
+    if (rc) return rc;
+    if (!c->Kd) DALLOC(c->Kd, (size_t)c->ws_ld * c->ws_ld);
+    if (!c->eig && !(c->eig = eigen_create(st))) {
+        dh::set_error("dh_eigenpairs: rocBLAS handle creation failed");
+        return DH_ERR_HIP;
+    }
+    double *Z = nullptr, *Cd = nullptr, *Ed = nullptr;
+    int* ints = nullptr;
+    auto release = [&]() {
+        hipFree(Z); hipFree(Cd); hipFree(Ed); hipFree(ints);
+    };
+    // Z: all N columns, which dsyevdx may use as workspace whatever the range
+    if (dalloc(&Z, (size_t)ld * N) || dalloc(&Cd, (size_t)ld * nev) || dalloc(&Ed, (size_t)N) || dalloc(&ints, 2)) {
+        release();
+        return DH_ERR_NOMEM;
+    }
+    rc = eigen_pairs(c->eig, N, ld, s->Hth, s->Q2, c->Kd, Z, Cd, Ed, il, iu, ints, st);
+    if (rc == DH_OK && E)
+        if (hipMemcpyAsync(E, Ed, sizeof(double) * nev, hipMemcpyDeviceToHost, st) != hipSuccess) rc = DH_ERR_HIP;
+    if (rc == DH_OK && C) rc = copy_block(C, ldc, Cd, ld, N, nev, hipMemcpyDeviceToHost, st) ? DH_ERR_HIP : DH_OK;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == DH_OK) rc = DH_ERR_HIP;
+    release();
+    if (rc == DH_ERR_HIP && !*dh::last_error()) dh::set_error("dh_eigenpairs: device copy failed");
+    return rc;
+}
+
+// Wavepacket wp <- eigenvector k (1-based) of (h, S), as FMO seeds the electron
+// and the hole with the whole-system LUMO and HOMO (tuning.f:76-83; FMO.f:294-350),
+// then preprocess_ElHl_Chebyshev sets AO_bra = AO_ket = C (ElHl_Chebyshev.f:110-129).
+extern "C" int dh_set_eigenstate(dh_system* s, int wp, int k, double* energy) {
+    CHECK_ARG(s && wp >= 0 && wp < s->W && k >= 1 && k <= s->N, "dh_set_eigenstate: bad argument");
+    std::vector<double> cr(s->N), cz(2 * (size_t)s->N, 0.0);
+    double e = 0.0;
+    int rc = dh_eigenpairs(s, k, k, &e, cr.data(), s->N);
+    if (rc) return rc;
+    for (int i = 0; i < s->N; ++i) cz[2 * (size_t)i] = cr[i];
+    if (energy) *energy = e;
+    return dh_set_initial_state(s, wp, cz.data(), cz.data());
 }
 
 extern "C" int dh_set_initial_state(dh_system* s, int wp, const double* ao_bra, const double* ao_ket) {

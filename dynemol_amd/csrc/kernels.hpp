@@ -84,6 +84,15 @@ void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st);
 // dst (ld x ld) = lower triangle of src, zero above the diagonal
 void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st);
 
+// ---------------------------------------------------------------- initial-state eigenpairs
+// Eigenpairs il..iu (1-based, ascending) of h C = S C E from H_orth and L^-1
+// (Q2, lower triangle): E (device, N doubles) and C = L^-T V (device, ld x nev).
+struct EigenSolver;
+EigenSolver* eigen_create(hipStream_t st);
+void eigen_destroy(EigenSolver* es);
+int eigen_pairs(EigenSolver* es, int N, int ld, const double* Hth, const double* Q2, double* A, double* Z, double* C,
+                double* E, int il, int iu, int* d_ints, hipStream_t st);
+
 // Cholesky S = L L^T in place (lower; upper zeroed) on an ld x ld padded
 // matrix, plus L^-1 in Linv.  info (device int) gets the first failing
 // column + 1, or stays 0.  Dinv: (ld/64) blocks of 64x64 workspace.

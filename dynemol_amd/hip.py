@@ -35,7 +35,7 @@ SYMBOLS = [
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
-    "dh_set_graph", "dh_ehrenfest_kernel",
+    "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read"]
@@ -86,6 +86,8 @@ def lib():
     L.dh_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
     L.dh_get_observables.argtypes = [_vp, ctypes.c_int, _d, _d, _d]
     L.dh_ehrenfest_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int]
+    L.dh_eigenpairs.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, _d, ctypes.c_int]
+    L.dh_set_eigenstate.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d]
     L.dh_get_stats.argtypes = [_vp, _d]
     L.dh_propagation_elhl.argtypes = [_vp, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]
     L.dh_test_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int, _d,
@@ -244,6 +246,21 @@ class System:
         k = np.ascontiguousarray(ao_ket, dtype=np.complex128)
         _check(lib().dh_set_initial_state(self.h, wp, b.ctypes.data_as(_d), k.ctypes.data_as(_d)),
                "dh_set_initial_state")
+
+    def eigenpairs(self, il, iu, vectors=True):
+        """Eigenpairs il..iu (1-based, ascending) of h C = S C E on the device: (E, C)."""
+        n = iu - il + 1
+        E = np.zeros(n)
+        C = np.zeros((self.N, n), order="F") if vectors else None
+        _check(lib().dh_eigenpairs(self.h, il, iu, _p(E), None if C is None else C.ctypes.data_as(_d), self.N),
+               "dh_eigenpairs")
+        return E, C
+
+    def set_eigenstate(self, wp, k):
+        """Seed wavepacket wp with eigenvector k (1-based) of (h, S); returns E_k."""
+        e = ctypes.c_double(0.0)
+        _check(lib().dh_set_eigenstate(self.h, wp, k, ctypes.byref(e)), "dh_set_eigenstate")
+        return e.value
 
     def wavepacket(self, wp):
         b = np.zeros(self.N, dtype=np.complex128)

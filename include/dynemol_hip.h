@@ -104,6 +104,19 @@ int dh_get_wavepacket(dh_system *sys, int wp, double *psi_bra, double *psi_ket);
  * forms them (ElHl_Chebyshev.f:110-129); also computes the t = 0 observables.
  * Needs built operators. */
 int dh_set_initial_state(dh_system *sys, int wp, const double *ao_bra, const double *ao_ket);
+/* Eigenpairs il..iu (1-based, ascending) of the generalised problem
+ * h C = S C E of the system's current operators, computed on the device
+ * (eigenpairs of H_orth, then C = L^-T V, so C^T S C = I): E (iu - il + 1,
+ * eV) and C (N x (iu - il + 1), column-major, ldc >= N); either may be NULL.
+ * The sign of each column is arbitrary, as with any eigensolver.  Replaces
+ * the dsygvd solve of FMO (FMO.f:294-350; xPU_dsygvd, GPU_Interface.cpp:705-853)
+ * for whole-system orbitals. */
+int dh_eigenpairs(dh_system *sys, int il, int iu, double *E, double *C, int ldc);
+/* Seed wavepacket wp with eigenvector k (1-based) of (h, S): AO_bra = AO_ket
+ * = C_k, as the electron (LUMO) and hole (HOMO) start when all atoms belong to
+ * both fragments (tuning.f:76-83, ElHl_Chebyshev.f:110-129).  energy (may be
+ * NULL) receives E_k.  Then as dh_set_initial_state. */
+int dh_set_eigenstate(dh_system *sys, int wp, int k, double *energy);
 /*
  * Propagate every wavepacket from t_init to t_max (ps): Psi_ket <- exp(-i
  * S^-1 h dt / hbar) Psi_ket and Psi_bra <- exp(-i h S^-1 dt / hbar) Psi_bra,
