@@ -1607,14 +1607,14 @@ extern "C" int dh_eigenpairs(dh_system* s, int il, int iu, double* E, double* C,
     auto release = [&]() {
         hipFree(Z); hipFree(Cd); hipFree(Ed); hipFree(ints);
     };
-    // Z: all N columns, which dsyevdx may use as workspace whatever the range
-    if (dalloc(&Z, (size_t)ld * N) || dalloc(&Cd, (size_t)ld * nev) || dalloc(&Ed, (size_t)N) || dalloc(&ints, 2)) {
+    if (dalloc(&Z, (size_t)ld * ld) || dalloc(&Cd, (size_t)ld * nev) || dalloc(&Ed, 2 * (size_t)N) || dalloc(&ints, 2)) {
         release();
         return DH_ERR_NOMEM;
     }
     rc = eigen_pairs(c->eig, N, ld, s->Hth, s->Q2, c->Kd, Z, Cd, Ed, il, iu, ints, st);
     if (rc == DH_OK && E)
-        if (hipMemcpyAsync(E, Ed, sizeof(double) * nev, hipMemcpyDeviceToHost, st) != hipSuccess) rc = DH_ERR_HIP;
+        if (hipMemcpyAsync(E, Ed + (il - 1), sizeof(double) * nev, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = DH_ERR_HIP;
     if (rc == DH_OK && C) rc = copy_block(C, ldc, Cd, ld, N, nev, hipMemcpyDeviceToHost, st) ? DH_ERR_HIP : DH_OK;
     if (hipStreamSynchronize(st) != hipSuccess && rc == DH_OK) rc = DH_ERR_HIP;
     release();

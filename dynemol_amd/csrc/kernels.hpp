@@ -86,7 +86,8 @@ void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st);
 
 // ---------------------------------------------------------------- initial-state eigenpairs
 // Eigenpairs il..iu (1-based, ascending) of h C = S C E from H_orth and L^-1
-// (Q2, lower triangle): E (device, N doubles) and C = L^-T V (device, ld x nev).
+// (Q2, lower triangle): E (device, 2 N doubles; eigenvalue j at E[j - 1]) and
+// C = L^-T V (device, ld x nev); A and Z are ld x ld scratch.
 struct EigenSolver;
 EigenSolver* eigen_create(hipStream_t st);
 void eigen_destroy(EigenSolver* es);

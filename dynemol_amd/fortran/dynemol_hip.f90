@@ -95,6 +95,18 @@ module dynemol_hip
             complex(c_double_complex) :: ao_bra(*)
             real(c_double) :: pops(*), erg(2)
         end function
+        ! eigenpairs il..iu (1-based) of h C = S C E; E and C may be c_null_ptr
+        integer(c_int) function dh_eigenpairs(sys, il, iu, E, C, ldc) bind(C, name="dh_eigenpairs")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: sys, E, C
+            integer(c_int), value :: il, iu, ldc
+        end function
+        integer(c_int) function dh_set_eigenstate(sys, wp, k, energy) bind(C, name="dh_set_eigenstate")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: sys
+            integer(c_int), value :: wp, k
+            real(c_double) :: energy
+        end function
         ! X: c_loc of real*8 X(ldx, N), or c_null_ptr to form X_ij from the system's basis
         integer(c_int) function dh_ehrenfest_kernel(sys, wp_el, wp_hl, X, ldx, K, ldk) bind(C, name="dh_ehrenfest_kernel")
             import :: c_int, c_double, c_ptr
