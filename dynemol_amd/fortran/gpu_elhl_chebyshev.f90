@@ -33,6 +33,12 @@ module gpu_ElHl_Chebyshev_m
     private
     public :: gpu_preprocess_ElHl_Chebyshev, gpu_ElHl_Chebyshev, gpu_ElHl_finalize, QM_wp_erg, gpu_EhrenfestKernel
 
+    ! a fresh start or a restart, told apart by the argument list, as in the
+    ! reference's generic (ElHl_Chebyshev_GPU.f:45-48)
+    interface gpu_preprocess_ElHl_Chebyshev
+        module procedure preprocess_fresh, preprocess_from_restart
+    end interface
+
     real*8, parameter :: frame_step = 1.d0      ! parameters_m frame_step for slice_Cheb
     type(c_ptr), save :: ctx = c_null_ptr, sysh = c_null_ptr
     logical, save :: first_call = .true.
@@ -50,22 +56,17 @@ contains
         end if
     end subroutine
 
-    ! preprocess_ElHl_Chebyshev (ElHl_Chebyshev.f:50-143) / gpu_preprocess (ElHl_Chebyshev_GPU.f:55-160)
-    subroutine gpu_preprocess_ElHl_Chebyshev(system, basis, AO_bra, AO_ket, Dual_bra, Dual_ket, QDyn, it)
+    ! The device system of the electron and the hole, its S and h built from the
+    ! coordinates (Overlap_Matrix + Build_Huckel of both preprocess routines)
+    subroutine create_system(system, basis, nf)
         type(structure), intent(in) :: system
         type(STO_basis), intent(in) :: basis(:)
-        complex*16, intent(inout) :: AO_bra(:, :), AO_ket(:, :)
-        complex*16, intent(out) :: Dual_bra(:, :), Dual_ket(:, :)
-        type(f_time), intent(inout) :: QDyn
-        integer, intent(in) :: it
-        integer :: N, i, a, p, nf
+        integer, intent(in) :: nf
+        integer :: N, i, a
         integer(c_int), allocatable :: first_orb(:), lmax(:), on(:), ol(:), om(:), onz(:), ofr(:)
         real(c_double), allocatable :: oz(:), oc(:), oip(:), okwh(:), ovs(:), xyz(:)
-        complex*16, allocatable :: psi_bra(:), psi_ket(:), ao(:)
-        real*8 :: pops(QDyn%n_fragments + 1), erg(2)
 
         N = size(basis)
-        nf = QDyn%n_fragments
         allocate(first_orb(system%atoms + 1), lmax(system%atoms), on(N), ol(N), om(N), onz(N), ofr(N))
         allocate(oz(2 * N), oc(2 * N), oip(N), okwh(N), ovs(N), xyz(3 * system%atoms))
         lmax = 0
@@ -90,8 +91,25 @@ contains
         end do
         call check(dh_set_coords(sysh, xyz), 'dh_set_coords')
         call check(dh_build_operators(sysh, 12.d0, 0.d0), 'dh_build_operators')
+        if (allocated(last_coord)) deallocate(last_coord)
         allocate(last_coord, source=system%coord)
+    end subroutine create_system
 
+    ! preprocess_ElHl_Chebyshev (ElHl_Chebyshev.f:50-143) / gpu_preprocess (ElHl_Chebyshev_GPU.f:55-160)
+    subroutine preprocess_fresh(system, basis, AO_bra, AO_ket, Dual_bra, Dual_ket, QDyn, it)
+        type(structure), intent(in) :: system
+        type(STO_basis), intent(in) :: basis(:)
+        complex*16, intent(inout) :: AO_bra(:, :), AO_ket(:, :)
+        complex*16, intent(out) :: Dual_bra(:, :), Dual_ket(:, :)
+        type(f_time), intent(inout) :: QDyn
+        integer, intent(in) :: it
+        integer :: N, p, nf
+        complex*16, allocatable :: psi_bra(:), psi_ket(:), ao(:)
+        real*8 :: pops(QDyn%n_fragments + 1), erg(2)
+
+        N = size(basis)
+        nf = QDyn%n_fragments
+        call create_system(system, basis, nf)
         allocate(psi_bra(N), psi_ket(N), ao(N))
         do p = 1, 2
             call check(dh_set_initial_state(sysh, int(p - 1, c_int), AO_bra(:, p), AO_ket(:, p)), 'dh_set_initial_state')
@@ -107,7 +125,35 @@ contains
             QM_wp_erg(p) = cmplx(erg(1), erg(2), kind=8)
         end do
         first_call = .true.
-    end subroutine gpu_preprocess_ElHl_Chebyshev
+    end subroutine preprocess_fresh
+
+    ! preprocess_from_restart (ElHl_Chebyshev.f:377-433): the states of a
+    ! Restart_copy.dat snapshot (Restart_State_Cheb, backup.f:405-445) become
+    ! Psi_bra = DUAL_ket and Psi_ket = AO_ket; QM_wp_erg follows from them.
+    subroutine preprocess_from_restart(system, basis, DUAL_ket, AO_bra, AO_ket)
+        type(structure), intent(in) :: system
+        type(STO_basis), intent(in) :: basis(:)
+        complex*16, intent(in) :: DUAL_ket(:, :), AO_bra(:, :), AO_ket(:, :)
+        integer :: N, p, nf
+        integer(c_int) :: nt
+        complex*16, allocatable :: ao(:)
+        real*8, allocatable :: pops(:)
+        real*8 :: erg(2)
+
+        N = size(basis)
+        nf = maxval(basis%fragment)
+        call create_system(system, basis, nf)
+        allocate(ao(N), pops(nf + 1))
+        do p = 1, 2
+            call check(dh_set_wavepacket(sysh, int(p - 1, c_int), DUAL_ket(:, p), AO_ket(:, p)), 'dh_set_wavepacket')
+        end do
+        call check(dh_propagate(sysh, 0.d0, 0.d0, nt), 'dh_propagate')  ! zero step: the outputs of the restored states
+        do p = 1, 2
+            call check(dh_get_observables(sysh, int(p - 1, c_int), ao, pops, erg), 'dh_get_observables')
+            QM_wp_erg(p) = cmplx(erg(1), erg(2), kind=8)
+        end do
+        first_call = .true.
+    end subroutine preprocess_from_restart
 
     ! ElHl_Chebyshev (ElHl_Chebyshev.f:148-291) / gpu_ElHl_Chebyshev (ElHl_Chebyshev_GPU.f:168-318)
     subroutine gpu_ElHl_Chebyshev(system, basis, AO_bra, AO_ket, Dual_bra, Dual_ket, QDyn, t, delta_t, it)

@@ -57,3 +57,34 @@ def test_fortran_harness_matches_oracle(tmp_path, rebuild_every):
     Hp = O.dgemm(O.syinvert(Sf), O.huckel(b, Sf))
     Kref, _ = O.ehrenfest_kernel(Hp, O.huckel_x(b), np.conj(np.linalg.solve(Sf, ref.psi_bra)), ref.psi_ket)
     assert K is not None and np.abs(K - Kref).max() <= 1e-8 * np.abs(Kref).max()
+
+
+@pytest.mark.parametrize("rebuild_every", [0, 3])
+def test_fortran_restart_matches_straight_run(tmp_path, rebuild_every):
+    """Dropping the device state before step 6 and resuming from DUAL_ket / AO_bra /
+    AO_ket through the restart form of gpu_preprocess_ElHl_Chebyshev continues the
+    run: populations to 1e-10 and states to 1e-9 of the uninterrupted run."""
+    print(f"[gpu] fortran harness restart rebuild_every={rebuild_every}", flush=True)
+    st = ch_cluster(seed=1)
+    b = st.basis()
+    S = O.overlap(st.coords, b)
+    h = O.huckel(b, S)
+    E, C = sl.eigh(h, S)
+    nsteps = 9
+    rng = np.random.default_rng(10)
+    disp = [rng.normal(0, 0.02, st.coords.shape) for _ in range(nsteps // max(1, rebuild_every))] if rebuild_every else []
+    deck = tmp_path / "deck.txt"
+    write_deck(str(deck), st, b, C[:, 208], C[:, 207], nsteps, DT, rebuild_every, disp)
+    outs = []
+    for extra in ([], ["6"]):
+        out = tmp_path / f"out{len(extra)}.txt"
+        r = subprocess.run([HARNESS, str(deck), str(out), *extra], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(read_output(str(out), b.N, 2, nsteps))
+    (d0, b0, k0, e0, K0), (d1, b1, k1, e1, K1) = outs
+    assert np.abs(d1 - d0).max() < 1e-10
+    for p in range(2):
+        assert np.abs(k1[p] - k0[p]).max() <= 1e-9 * np.abs(k0[p]).max()
+        assert np.abs(b1[p] - b0[p]).max() <= 1e-9 * np.abs(b0[p]).max()
+    assert np.abs(e1 - e0).max() < 1e-8
+    assert np.abs(K1 - K0).max() <= 1e-8 * np.abs(K0).max()
