@@ -166,10 +166,19 @@ def time_ehrenfest(sysm, reps=5):
     t0 = time.perf_counter()
     sysm.ehrenfest_kernel(0, 1)
     host_s = time.perf_counter() - t0
-    log(f"ehrenfest kernel N={N}: {1e3 * dev_s:.2f} ms on the device, {1e3 * host_s:.2f} ms with K to the host")
+    # the reference's CPU branch is K = X*A - matmul(H', A) (diabatic-Ehren.f:112-116): the same
+    # arithmetic with the host BLAS behind numpy, on matrices of the same size
+    rng = np.random.default_rng(0)
+    Hh, Ah, Xh = (np.asfortranarray(rng.normal(size=(N, N))) for _ in range(3))
+    t0 = time.perf_counter()
+    _ = Xh * Ah - Hh @ Ah
+    cpu_s = time.perf_counter() - t0
+    log(f"ehrenfest kernel N={N}: {1e3 * dev_s:.2f} ms on the device, {1e3 * host_s:.2f} ms with K to the host, "
+        f"{1e3 * cpu_s:.0f} ms with numpy on the host")
     return {"n_orbitals": N, "device_ms": round(1e3 * dev_s, 3), "with_host_copy_ms": round(1e3 * host_s, 3),
             "first_call_ms": round(1e3 * first_s, 3), "gemm_tflops": round(2.0 * N ** 3 / dev_s / 1e12, 2),
-            "fp64_mfma_peak_tflops": 78.6, "K_S_frobenius_rel": inv}
+            "fp64_mfma_peak_tflops": 78.6, "K_S_frobenius_rel": inv,
+            "host_numpy_ms": round(1e3 * cpu_s, 1), "host_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 SYM_VARIANTS = (16, 17, 18, 19, 20)  # symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4)
