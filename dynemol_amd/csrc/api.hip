@@ -35,7 +35,7 @@ struct dh_context {
     int ws_ld = 0;
     double *S = nullptr, *h = nullptr, *W = nullptr, *Hp = nullptr, *Dinv = nullptr;
     double* Kd = nullptr;  // Ehrenfest kernel matrix / eigensolver scratch (allocated on first use)
-    EigenSolver* eig = nullptr;  // dense eigensolver handle (created on first use)
+    LibHandle* lib = nullptr;  // rocBLAS handle on the context stream (created on first use)
     int* d_info = nullptr;
     const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
     dh_system* dropin = nullptr;
@@ -198,7 +198,7 @@ extern "C" int dh_finalize(dh_context* c) {
     if (c->dropin) dh_system_destroy(c->dropin);
     hipStreamSynchronize(c->stream);
     free_ws(c);
-    eigen_destroy(c->eig);
+    lib_destroy(c->lib);
     hipFree(c->d_info);
     hipStreamDestroy(c->stream);
     delete c;
@@ -1571,7 +1571,11 @@ extern "C" int dh_ehrenfest_kernel(dh_system* s, int wp_el, int wp_hl, const dou
     e.X = X ? c->Kd : nullptr; e.ldx = ld;
     e.ip = s->bd.ip; e.kwh = s->bd.kwh; e.vshift = s->bd.vshift;
     e.A = c->W; e.K = c->Kd;
-    launch_ehrenfest(e, c->Hp, st);
+    if (!c->lib && !(c->lib = lib_create(st))) {
+        dh::set_error("dh_ehrenfest_kernel: rocBLAS handle creation failed");
+        return DH_ERR_HIP;
+    }
+    if (int rc = launch_ehrenfest(e, c->Hp, c->lib, st)) return rc;
     DH_HIP(hipGetLastError());
     if (K) {
         int rc = copy_matrix(K, ldk, c->Kd, ld, s->N, hipMemcpyDeviceToHost, st);
@@ -1598,7 +1602,7 @@ extern "C" int dh_eigenpairs(dh_system* s, int il, int iu, double* E, double* C,
     int rc = ensure_ws(c, ld);
     if (rc) return rc;
     if (!c->Kd) DALLOC(c->Kd, (size_t)c->ws_ld * c->ws_ld);
-    if (!c->eig && !(c->eig = eigen_create(st))) {
+    if (!c->lib && !(c->lib = lib_create(st))) {
         dh::set_error("dh_eigenpairs: rocBLAS handle creation failed");
         return DH_ERR_HIP;
     }
@@ -1611,7 +1615,7 @@ extern "C" int dh_eigenpairs(dh_system* s, int il, int iu, double* E, double* C,
         release();
         return DH_ERR_NOMEM;
     }
-    rc = eigen_pairs(c->eig, N, ld, s->Hth, s->Q2, c->Kd, Z, Cd, Ed, il, iu, ints, st);
+    rc = eigen_pairs(c->lib, N, ld, s->Hth, s->Q2, c->Kd, Z, Cd, Ed, il, iu, ints, st);
     if (rc == DH_OK && E)
         if (hipMemcpyAsync(E, Ed + (il - 1), sizeof(double) * nev, hipMemcpyDeviceToHost, st) != hipSuccess)
             rc = DH_ERR_HIP;

@@ -7,8 +7,8 @@
 //   K      = X o A - H' A,   X_ij the extended-Hueckel factors (h_ij = X_ij S_ij)
 //
 // One elementwise pass writes A and X o A (the latter into K); the N^3 product
-// is the library's FP64 matrix-core GEMM with C = X o A in the accumulators
-// (alpha = -1, beta = 1).
+// K <- K - H' A is rocBLAS dgemm (alpha = -1, beta = 1), or the library's FP64
+// matrix-core GEMM with C = X o A in the accumulators when no handle is given.
 #include "kernels.hpp"
 
 namespace dh {
@@ -84,9 +84,10 @@ void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st) {
     hipLaunchKernelGGL(copy_lower_kernel, dim3((ld + 255) / 256, ld), dim3(256), 0, st, src, dst, ld);
 }
 
-void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st) {
+int launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, LibHandle* lib, hipStream_t st) {
     hipLaunchKernelGGL(ehrenfest_prep_kernel, dim3((e.ld + 255) / 256, (e.ld + EH_JB - 1) / EH_JB), dim3(256), 0, st,
                        e);
+    if (lib) return lib_dgemm_nn(lib, e.N, e.N, e.N, -1.0, Hp, e.ld, e.A, e.ld, 1.0, e.K, e.ld);
     Gemm g{};
     g.M = e.N; g.N = e.N; g.K = e.N;
     g.A = Hp; g.lda = e.ld;
@@ -94,6 +95,7 @@ void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st) 
     g.C = e.K; g.ldc = e.ld;
     g.alpha = -1.0; g.beta = 1.0; g.kmode = K_FULL; g.batch = 1;
     launch_gemm(g, st);
+    return 0;
 }
 
 }  // namespace dh

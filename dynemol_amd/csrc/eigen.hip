@@ -1,6 +1,11 @@
-// Generalised eigenpairs h C = S C E (C^T S C = I) of a system's resident
-// operators, for initial states: the reference's FMO step solves the same
-// pencil with LAPACK/MAGMA dsygvd (FMO.f:294-350, GPU_Interface.cpp:705-853).
+// Library calls of the build, through one rocBLAS handle per context:
+// * generalised eigenpairs h C = S C E (C^T S C = I) of a system's resident
+//   operators, for initial states: the reference's FMO step solves the same
+//   pencil with LAPACK/MAGMA dsygvd (FMO.f:294-350, GPU_Interface.cpp:705-853);
+// * the plain full FP64 GEMM of the Ehrenfest kernel matrix (rocBLAS dgemm ran
+//   the 4096^3 product in 2.2 ms on this part, the library's own MFMA kernel
+//   in 2.8 ms; the rebuild's triangular products stay on the library's
+//   kernels, which skip the zero half).
 //
 // H_orth = L^-1 h L^-T is symmetric with the pencil's eigenvalues, so the
 // eigenpairs (ascending) come from rocSOLVER's dsyevd on the device, and
@@ -14,12 +19,12 @@
 
 namespace dh {
 
-struct EigenSolver {
+struct LibHandle {
     rocblas_handle h = nullptr;
 };
 
-EigenSolver* eigen_create(hipStream_t st) {
-    auto* es = new EigenSolver;
+LibHandle* lib_create(hipStream_t st) {
+    auto* es = new LibHandle;
     if (rocblas_create_handle(&es->h) != rocblas_status_success) {
         delete es;
         return nullptr;
@@ -28,10 +33,21 @@ EigenSolver* eigen_create(hipStream_t st) {
     return es;
 }
 
-void eigen_destroy(EigenSolver* es) {
+void lib_destroy(LibHandle* es) {
     if (!es) return;
     if (es->h) rocblas_destroy_handle(es->h);
     delete es;
+}
+
+int lib_dgemm_nn(LibHandle* es, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+                 double beta, double* C, int ldc) {
+    const rocblas_status rs = rocblas_dgemm(es->h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A,
+                                            lda, B, ldb, &beta, C, ldc);
+    if (rs != rocblas_status_success) {
+        set_error("rocblas_dgemm returned status %d", (int)rs);
+        return DH_ERR_HIP;
+    }
+    return DH_OK;
 }
 
 // All eigenpairs of H_orth by divide and conquer (dsyevd: 0.15 s at n = 4096
@@ -42,7 +58,7 @@ void eigen_destroy(EigenSolver* es) {
 // selected eigenvalues are E[il - 1 .. iu - 1].  Q2 holds L^-1 in its lower
 // triangle (its upper triangle is not read).  Returns 0, or sets the error
 // text and returns a DH_ERR code.
-int eigen_pairs(EigenSolver* es, int N, int ld, const double* Hth, const double* Q2, double* A, double* Z, double* C,
+int eigen_pairs(LibHandle* es, int N, int ld, const double* Hth, const double* Q2, double* A, double* Z, double* C,
                 double* E, int il, int iu, int* d_ints, hipStream_t st) {
     const int nev = iu - il + 1;
     launch_copy_matrix(Hth, A, (long)ld * ld, st);

@@ -60,6 +60,8 @@ struct Gemm {
 };
 void launch_gemm(const Gemm& g, hipStream_t st);
 
+struct LibHandle;  // rocBLAS handle (below)
+
 // ---------------------------------------------------------------- diabatic Ehrenfest
 // A = (rho + rho^T)/2 with rho_ij = Re(ket_j,el bra_i,el) - Re(ket_j,hl bra_i,hl),
 // then K = X o A - H' A.  bra and ket are row-major complex states (row r,
@@ -80,18 +82,23 @@ struct EhrenfestArgs {
     double* A;
     double* K;
 };
-void launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, hipStream_t st);
+// lib: the product goes to rocBLAS dgemm (else the library's MFMA GEMM)
+int launch_ehrenfest(const EhrenfestArgs& e, const double* Hp, LibHandle* lib, hipStream_t st);
 // dst (ld x ld) = lower triangle of src, zero above the diagonal
 void launch_copy_lower(const double* src, double* dst, int ld, hipStream_t st);
 
-// ---------------------------------------------------------------- initial-state eigenpairs
+// ---------------------------------------------------------------- library calls (rocBLAS / rocSOLVER)
+// One handle per context, on the context stream.
+struct LibHandle;
+LibHandle* lib_create(hipStream_t st);
+void lib_destroy(LibHandle* es);
+// C = alpha A B + beta C, column-major, no transposes
+int lib_dgemm_nn(LibHandle* es, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+                 double beta, double* C, int ldc);
 // Eigenpairs il..iu (1-based, ascending) of h C = S C E from H_orth and L^-1
 // (Q2, lower triangle): E (device, 2 N doubles; eigenvalue j at E[j - 1]) and
 // C = L^-T V (device, ld x nev); A and Z are ld x ld scratch.
-struct EigenSolver;
-EigenSolver* eigen_create(hipStream_t st);
-void eigen_destroy(EigenSolver* es);
-int eigen_pairs(EigenSolver* es, int N, int ld, const double* Hth, const double* Q2, double* A, double* Z, double* C,
+int eigen_pairs(LibHandle* es, int N, int ld, const double* Hth, const double* Q2, double* A, double* Z, double* C,
                 double* E, int il, int iu, int* d_ints, hipStream_t st);
 
 // Cholesky S = L L^T in place (lower; upper zeroed) on an ld x ld padded
