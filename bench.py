@@ -30,13 +30,15 @@ HBM_PEAK_GBS = 8000.0
 # 4 RHS) from rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by what the same
 # counter reads for the library's plain copy kernel over 256 MiB (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000): profiles/cheb_term_sym_tio2_N4096_counters.md.
-# Variants 17 and 18 likewise: profiles/cheb_term_symm_tio2_N4096_counters.md and
-# profiles/cheb_term_symm4_tio2_N4096_counters.md.
+# Variants 17, 18 and 20 likewise: profiles/cheb_term_symm_tio2_N4096_counters.md,
+# profiles/cheb_term_symm4_tio2_N4096_counters.md and profiles/cheb_term_symm3_tio2_N4096_counters.md.
 MEASURED_TRAFFIC_BYTES = {16: int(((35409.8125 + 5487.5) * 2.0 + (10924.0 + 256.0)) * 1024),
                           17: int(((34796.36781939338 + 5874.9525547445255) * 2.0
                                    + (10924.001321231617 + 508.26277372262774)) * 1024),
                           18: int(((34810.57783777574 + 5872.213768115942) * 2.0
-                                   + (10924.024931066177 + 506.4347826086956)) * 1024)}
+                                   + (10924.024931066177 + 506.4347826086956)) * 1024),
+                          20: int(((34719.24046415441 + 5869.5) * 2.0
+                                   + (10925.4521484375 + 504.63309352517985)) * 1024)}
 
 
 def log(msg):
