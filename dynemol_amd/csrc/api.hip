@@ -1296,12 +1296,67 @@ struct dh_batch {
     std::vector<int> Ks;
     int Kmax = 0;
     double stats[4] = {0, 0, 0, 0};  // steps, term launches, last max drift, retries
+    // the term series as one captured graph, kept while the term counts stay
+    // the same (its kernels read the per-step descriptors from d_args)
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int> gKs;
 };
+
+// Chebyshev terms of a batch: systems with K_s >= k are the first `active` of `order`
+static void batch_series_launches(dh_batch* b, hipStream_t st) {
+    const int n = b->n, R = b->R;
+    int active = n;
+    for (int k = 1; k <= b->Kmax; ++k) {
+        while (active > 0 && b->Ks[b->order[active - 1]] < k) --active;
+        const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
+        launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k, st);
+    }
+}
+
+static int batch_series(dh_batch* b, hipStream_t st) {
+    if (!b->use_graph) {
+        batch_series_launches(b, st);
+        return DH_OK;
+    }
+    if (!b->gexec || b->gKs != b->Ks) {
+        if (b->gexec) {
+            DH_HIP(hipGraphExecDestroy(b->gexec));
+            b->gexec = nullptr;
+        }
+        DH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        batch_series_launches(b, st);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            dh::set_error("batch term series capture failed: %s", hipGetErrorString(e));
+            return DH_ERR_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            b->gexec = nullptr;
+            dh::set_error("batch term series graph instantiation failed: %s", hipGetErrorString(ei));
+            return DH_ERR_HIP;
+        }
+        b->gKs = b->Ks;
+    }
+    DH_HIP(hipGraphLaunch(b->gexec, st));
+    return DH_OK;
+}
+
+extern "C" int dh_batch_set_graph(dh_batch* b, int on) {
+    CHECK_ARG(b, "dh_batch_set_graph: null batch");
+    b->use_graph = on != 0;
+    return DH_OK;
+}
 
 extern "C" int dh_batch_destroy(dh_batch* b) {
     if (!b) return DH_OK;
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
     void* ptrs[] = {b->d_args, b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs, b->d_nval};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -1456,13 +1511,7 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     DH_HIP(hipMemcpyAsync(b->d_args, args.data(), sizeof(MatvecArgs) * args.size(), hipMemcpyHostToDevice, st));
     DH_HIP(hipMemcpyAsync(b->d_norms, nt.data(), sizeof(NormTask) * nt.size(), hipMemcpyHostToDevice, st));
     launch_grouped_norms(b->d_norms, n * R, st);
-    // Chebyshev terms: systems with K_s >= k are the first `active` of `order`
-    int active = n;
-    for (int k = 1; k <= b->Kmax; ++k) {
-        while (active > 0 && Ks[b->order[active - 1]] < k) --active;
-        const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
-        launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k, st);
-    }
+    if (int rc = batch_series(b, st)) return rc;
     launch_grouped_norms(b->d_norms + (size_t)n * R, n * R, st);
     std::vector<double> nv((size_t)2 * n * R);
     DH_HIP(hipMemcpyAsync(nv.data(), b->d_nval, sizeof(double) * nv.size(), hipMemcpyDeviceToHost, st));

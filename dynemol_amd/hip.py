@@ -35,7 +35,7 @@ SYMBOLS = [
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
-    "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate",
+    "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read"]
@@ -87,6 +87,7 @@ def lib():
     L.dh_get_observables.argtypes = [_vp, ctypes.c_int, _d, _d, _d]
     L.dh_ehrenfest_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int]
     L.dh_eigenpairs.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, _d, ctypes.c_int]
+    L.dh_batch_set_graph.argtypes = [_vp, ctypes.c_int]
     L.dh_set_eigenstate.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d]
     L.dh_get_stats.argtypes = [_vp, _d]
     L.dh_propagation_elhl.argtypes = [_vp, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d]
@@ -364,6 +365,10 @@ class Batch:
         n = ctypes.c_int(0)
         _check(lib().dh_batch_propagate(self.h, t_init, t_max, ctypes.byref(n)), "dh_batch_propagate")
         return n.value
+
+    def set_graph(self, on=True):
+        """Term series as one captured hipGraph per step (default) or as separate launches."""
+        _check(lib().dh_batch_set_graph(self.h, 1 if on else 0), "dh_batch_set_graph")
 
     def close(self):
         if self.h:

@@ -177,6 +177,11 @@ typedef struct dh_batch dh_batch;
 int dh_batch_create(dh_context *ctx, int n, dh_system *const *systems, dh_batch **batch);
 int dh_batch_destroy(dh_batch *batch);
 int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_terms);
+/* Run each step's Chebyshev term launches as one captured hipGraph (on = 1,
+ * the default; captured again when a system's term count changes) or as
+ * individual launches (on = 0).  Results are identical either way.  No
+ * counterpart in the reference. */
+int dh_batch_set_graph(dh_batch *batch, int on);
 
 /*
  * Drop-in for propagationelhl_gpucaller_ (Taylor_gpu.cpp:219-232, 634-736):

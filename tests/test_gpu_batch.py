@@ -65,3 +65,36 @@ def test_batch_matches_single_and_exact():
                 assert np.abs(pb - ps).max() < 1e-12
                 assert abs(eb - es) < 1e-10
         batch.close()
+
+
+def test_batch_graph_replays_match_launches():
+    """The captured term series (default) against individual launches, bitwise, over
+    steps that replay one graph, with host reads between the replays, and a step of
+    another length that captures a new one."""
+    sizes = [300, 517, 256, 700]
+    _say(f"batch graph vs launches, N={sizes}")
+    data = [_system_inputs(200 + i, n) for i, n in enumerate(sizes)]
+    with hip.Context(0) as ctx:
+        groups = {True: [], False: []}
+        for st, b, S, E, C, kets in data:
+            for on in (True, False):
+                s = hip.System(ctx, b, n_wp=4, n_frag=2)
+                s.build(st.coords)
+                for w, k in enumerate(kets):
+                    s.set_wavepacket(w, S @ k, k)
+                groups[on].append(s)
+        batches = {on: hip.Batch(ctx, groups[on]) for on in (True, False)}
+        batches[False].set_graph(False)
+        t = 0.0
+        for dt in (DT, DT, DT, 0.5 * DT, DT):
+            for on in (True, False):
+                batches[on].propagate(t, t + dt)
+            t += dt
+            for sg, sl_ in zip(groups[True], groups[False]):
+                for w in range(4):
+                    bg, kg = sg.wavepacket(w)
+                    bl, kl = sl_.wavepacket(w)
+                    assert np.array_equal(kg, kl) and np.array_equal(bg, bl)
+                    assert np.array_equal(sg.observables(w)[1], sl_.observables(w)[1])
+        for on in (True, False):
+            batches[on].close()
