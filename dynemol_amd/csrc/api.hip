@@ -1301,6 +1301,12 @@ struct dh_batch {
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     std::vector<int> gKs;
+    // R in {2, 4, 8}: the terms run the symmetric half-storage kernel over all
+    // systems (phase-1 tasks: segments; phase-2 tasks: block x chunk), each
+    // system with the segment length its own single launch would use
+    bool sym = false;
+    int *d_t1_desc = nullptr, *d_t1_idx = nullptr, *d_t2_desc = nullptr, *d_t2_idx = nullptr, *d_sysS = nullptr;
+    std::vector<int> prefix1, prefix2;
 };
 
 // Chebyshev terms of a batch: systems with K_s >= k are the first `active` of `order`
@@ -1310,7 +1316,12 @@ static void batch_series_launches(dh_batch* b, hipStream_t st) {
     for (int k = 1; k <= b->Kmax; ++k) {
         while (active > 0 && b->Ks[b->order[active - 1]] < k) --active;
         const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
-        launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k, st);
+        if (b->sym)
+            launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
+                               b->d_t2_idx, b->prefix2[active], b->d_sysS, R, k, st);
+        else
+            launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k,
+                                  st);
     }
 }
 
@@ -1357,7 +1368,8 @@ extern "C" int dh_batch_destroy(dh_batch* b) {
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
-    void* ptrs[] = {b->d_args, b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs, b->d_nval};
+    void* ptrs[] = {b->d_args,  b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs,
+                    b->d_nval,  b->d_t1_desc,   b->d_t1_idx,   b->d_t2_desc, b->d_t2_idx, b->d_sysS};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete b;
@@ -1399,6 +1411,25 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
     BALLOC(b->d_copy, (size_t)n);
     BALLOC(b->d_obs, (size_t)n * W);
     BALLOC(b->d_nval, (size_t)2 * n * R);
+    b->sym = sym_supported(R, TRI_FULL);
+    if (b->sym) {
+        long t1 = 0, t2 = 0;
+        std::vector<int> sS(n);
+        for (int i = 0; i < n; ++i) {
+            t1 += sym_term_segments(b->sys[i]->ld, &sS[i]);
+            t2 += (long)(b->sys[i]->ld / TILE) * R;
+        }
+        BALLOC(b->d_t1_desc, (size_t)t1);
+        BALLOC(b->d_t1_idx, (size_t)t1);
+        BALLOC(b->d_t2_desc, (size_t)t2);
+        BALLOC(b->d_t2_idx, (size_t)t2);
+        BALLOC(b->d_sysS, (size_t)n);
+        if (hipMemcpy(b->d_sysS, sS.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            dh_batch_destroy(b);
+            dh::set_error("dh_batch_create: copy of the segment table failed");
+            return DH_ERR_HIP;
+        }
+    }
 #undef BALLOC
     *out = b;
     return DH_OK;
@@ -1411,6 +1442,7 @@ static MatvecArgs batch_desc(dh_system* s, int kind) {
     a.N = s->N;
     a.coef = s->coef;
     a.acc = s->acc;
+    a.partial = s->partial;  // the symmetric term's partial sums
     const double c = s->coef_c, r = s->coef_r;
     switch (kind) {
         case dh_batch::D_K1:
@@ -1495,6 +1527,30 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
         }
         DH_HIP(hipMemcpyAsync(b->d_task_desc, td.data(), sizeof(int) * td.size(), hipMemcpyHostToDevice, st));
         DH_HIP(hipMemcpyAsync(b->d_task_blk, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
+        if (b->sym) {  // the same system order for the symmetric term's two phases
+            std::vector<int> d1, i1, d2, i2;
+            b->prefix1.assign(1, 0);
+            b->prefix2.assign(1, 0);
+            for (int q : b->order) {
+                int S = 0;
+                const int nseg = sym_term_segments(b->sys[q]->ld, &S);
+                for (int g = 0; g < nseg; ++g) {
+                    d1.push_back(q);
+                    i1.push_back(g);
+                }
+                const int nbq = b->sys[q]->ld / TILE;
+                for (int v = 0; v < nbq * R; ++v) {
+                    d2.push_back(q);
+                    i2.push_back(v);
+                }
+                b->prefix1.push_back((int)d1.size());
+                b->prefix2.push_back((int)d2.size());
+            }
+            DH_HIP(hipMemcpyAsync(b->d_t1_desc, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t1_idx, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
+        }
         b->Kmax = Ks[b->order[0]];
     }
     // per-system descriptors (pointers are re-read every step)

@@ -1221,10 +1221,9 @@ constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per sourc
 static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
 
 template <int R>
-__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecArgs a, int S) {
+__device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int chunk) {
     __shared__ cplx red[SYM_G * 64];
     const int nb = a.ld / TILE;
-    const int B = blockIdx.x, chunk = blockIdx.y;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
     const int seg0 = sym_segment_start(B, S);
     const int nseg = sym_segments_in_row(B, S);
@@ -1269,6 +1268,24 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecAr
 }
 
 template <int R>
+__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecArgs a, int S) {
+    sym_reduce_body<R>(a, S, blockIdx.x, blockIdx.y);
+}
+
+// Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),
+// with the system's segment length sys_S[task_desc[i]]
+template <int R>
+__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_grouped_kernel(const MatvecArgs* __restrict__ args,
+                                                                                const int* __restrict__ task_desc,
+                                                                                const int* __restrict__ task_idx,
+                                                                                const int* __restrict__ sys_S, int k) {
+    const int d = task_desc[blockIdx.x], v = task_idx[blockIdx.x];
+    MatvecArgs a = args[d];
+    a.k = k;
+    sym_reduce_body<R>(a, sys_S[d], v / R, v % R);
+}
+
+template <int R>
 static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int nb = a.ld / TILE;
     const int S = sym_seg_for(nb);
@@ -1303,9 +1320,9 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
 // does no discarded work: ceil(2R / 4) products per k-step of 16 cycles each,
 // against one 64-cycle 16x16x4 whose rows >= 2R are discarded.
 template <int R, int MF, typename MT = double>
-__global__ __launch_bounds__(256) void symm_tiles_kernel(const MT* __restrict__ Mg, const double* __restrict__ xg,
-                                                         double* __restrict__ part, int ld, int N, int x_stride,
-                                                         int x_off, int S) {
+__device__ inline void symm_tiles_body(const MT* __restrict__ Mg, const double* __restrict__ xg,
+                                       double* __restrict__ part, int ld, int N, int x_stride, int x_off, int S,
+                                       int seg) {
     constexpr int HP = TILE + 1;
     constexpr int C2 = 2 * R;                   // real right-hand-side columns
     constexpr int XE = TILE * C2;               // doubles of one staged x block
@@ -1317,11 +1334,11 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const MT* __restrict__ 
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (sym_segment_start(mid, S) <= (int)blockIdx.x) lo = mid;
+        if (sym_segment_start(mid, S) <= seg) lo = mid;
         else hi = mid - 1;
     }
     const int I = lo;
-    const int Ja = ((int)blockIdx.x - sym_segment_start(I, S)) * S, Jb = min(I + 1, Ja + S);
+    const int Ja = (seg - sym_segment_start(I, S)) * S, Jb = min(I + 1, Ja + S);
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int i0 = I * TILE;
@@ -1421,18 +1438,64 @@ __global__ __launch_bounds__(256) void symm_tiles_kernel(const MT* __restrict__ 
     }
     const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
     if constexpr (MF == 0) {
-        double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + lr) * R);
+        double* py = part + 2 * (pcol + (long)seg * TILE * R + (long)(16 * w + lr) * R);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int m = lk + 4 * v;
             if (m < C2) py[m] = yacc[v];
         }
     } else {
-        double* py = part + 2 * (pcol + (long)blockIdx.x * TILE * R + (long)(16 * w + 4 * ((lane >> 2) & 3) + lk) * R);
+        double* py = part + 2 * (pcol + (long)seg * TILE * R + (long)(16 * w + 4 * ((lane >> 2) & 3) + lk) * R);
 #pragma unroll
         for (int h = 0; h < G; ++h)
             if (4 * h + xq < C2) py[4 * h + xq] = yacc4[h];
     }
+}
+
+template <int R, int MF, typename MT = double>
+__global__ __launch_bounds__(256) void symm_tiles_kernel(const MT* __restrict__ Mg, const double* __restrict__ xg,
+                                                         double* __restrict__ part, int ld, int N, int x_stride,
+                                                         int x_off, int S) {
+    symm_tiles_body<R, MF, MT>(Mg, xg, part, ld, N, x_stride, x_off, S, blockIdx.x);
+}
+
+// Batches: task i is segment task_idx[i] of system task_desc[i] (FP64 H, 4x4x4
+// products: the variant-18 arithmetic of each system, segment for segment)
+template <int R>
+__global__ __launch_bounds__(256) void symm_tiles_grouped_kernel(const MatvecArgs* __restrict__ args,
+                                                                 const int* __restrict__ task_desc,
+                                                                 const int* __restrict__ task_idx,
+                                                                 const int* __restrict__ sys_S) {
+    const int d = task_desc[blockIdx.x];
+    const MatvecArgs& a = args[d];
+    symm_tiles_body<R, 1, double>(a.M, reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial),
+                                  a.ld, a.N, a.x_stride, a.x_off, sys_S[d], task_idx[blockIdx.x]);
+}
+
+// segment length S and segment count of the 4x4x4 symmetric term at ld (the
+// grid of launch_symm below)
+int sym_term_segments(int ld, int* S) {
+    const int nb = ld / TILE;
+    *S = sym_seg_for(nb, 3);
+    return sym_segment_count(nb, *S);
+}
+
+void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, hipStream_t st) {
+    if (n1 > 0) switch (R) {
+            case 1: hipLaunchKernelGGL(symm_tiles_grouped_kernel<1>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
+            case 2: hipLaunchKernelGGL(symm_tiles_grouped_kernel<2>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
+            case 4: hipLaunchKernelGGL(symm_tiles_grouped_kernel<4>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
+            case 8: hipLaunchKernelGGL(symm_tiles_grouped_kernel<8>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
+            default: break;
+        }
+    if (n2 > 0) switch (R) {
+            case 1: hipLaunchKernelGGL(sym_reduce_grouped_kernel<1>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
+            case 2: hipLaunchKernelGGL(sym_reduce_grouped_kernel<2>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
+            case 4: hipLaunchKernelGGL(sym_reduce_grouped_kernel<4>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
+            case 8: hipLaunchKernelGGL(sym_reduce_grouped_kernel<8>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
+            default: break;
+        }
 }
 
 // MT = float: H's FP32 copy (a.M32), widened to FP64 on load (precision mode 2)
