@@ -32,15 +32,17 @@ def _system_inputs(seed, N, n_wp=4):
     return st, b, S, E, C, kets
 
 
-def test_batch_matches_single_and_exact():
+@pytest.mark.parametrize("n_wp", [4, 8])
+def test_batch_matches_single_and_exact(n_wp):
+    """4 wavepackets: one symmetric pass (R = 8); 8 wavepackets: two column halves (R = 16)."""
     sizes = [300, 517, 256, 800, 411, 1000]
-    _say(f"batch of {len(sizes)} systems, N={sizes}")
-    data = [_system_inputs(100 + i, n) for i, n in enumerate(sizes)]
+    _say(f"batch of {len(sizes)} systems, N={sizes}, n_wp={n_wp}")
+    data = [_system_inputs(100 + i, n, n_wp) for i, n in enumerate(sizes)]
     with hip.Context(0) as ctx:
         batched, single = [], []
         for st, b, S, E, C, kets in data:
             for group in (batched, single):
-                s = hip.System(ctx, b, n_wp=4, n_frag=2)
+                s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
                 s.build(st.coords)
                 for w, k in enumerate(kets):
                     s.set_wavepacket(w, S @ k, k)
@@ -53,7 +55,7 @@ def test_batch_matches_single_and_exact():
                 s.propagate(t, t + DT)
             t += DT
         for (st, b, S, E, C, kets), sb, ss in zip(data, batched, single):
-            for w in range(4):
+            for w in range(n_wp):
                 bb, kb = sb.wavepacket(w)
                 bs, ks = ss.wavepacket(w)
                 assert np.abs(kb - ks).max() <= 1e-11 * np.abs(ks).max()
