@@ -9,7 +9,8 @@
   Restart_State_Cheb (backup.f:329-445): a Fortran unformatted sequential file of
   records [frame], it, t, N, n_part, size(eh_tag), eh_tag(:), then per particle
   (DUAL_bra_i, DUAL_ket_i) and (AO_bra_i, AO_ket_i) interleaved over the
-  orbitals, then Net_Charge.  Records are framed by 4-byte little-endian
+  orbitals, then Net_Charge, put out size(Net_Charge) times over by the implied
+  DO of backup.f:386 (n_atoms**2 values in one record).  Records are framed by 4-byte little-endian
   lengths, the record layout of gfortran and of ifort (the reference's compiler)
   on x86-64.
 * `net_charge` is Net_Charge of Populations_mtx (data_output.f:132-138).
@@ -21,6 +22,7 @@
 """
 from __future__ import annotations
 
+import math
 import os
 import struct
 
@@ -123,7 +125,10 @@ def write_security_copy(path, dual_bra, dual_ket, ao_bra, ao_ket, t: float, it: 
     for j in range(n_part):
         recs.append(_interleave(dual_bra[:, j], dual_ket[:, j]))
         recs.append(_interleave(ao_bra[:, j], ao_ket[:, j]))
-    recs.append(np.asarray(net_charge, dtype="<f8").tobytes())
+    # write(33) ( Net_Charge , i=1,size(Net_Charge) ) (backup.f:386): the implied
+    # DO puts the whole array out size(Net_Charge) times, n_atoms**2 values in one record
+    q = np.asarray(net_charge, dtype="<f8").ravel()
+    recs.append(np.tile(q, len(q)).tobytes())
     with open(path, "wb") as f:
         for r in recs:
             f.write(_record(r))
@@ -151,7 +156,13 @@ def read_restart_copy(path, with_frame: bool = False) -> dict:
         arrays["dual_bra"][:, j], arrays["dual_ket"][:, j] = d[:, 0], d[:, 1]
         arrays["ao_bra"][:, j], arrays["ao_ket"][:, j] = a[:, 0], a[:, 1]
     out.update(arrays)
-    out["net_charge"] = np.frombuffer(recs[k + 6 + 2 * n_part], dtype="<f8").copy()
+    # read(33) ( Net_Charge , i=1,size(Net_Charge) ) (backup.f:436) reads the array
+    # size(Net_Charge) times over; what stays is the last copy
+    raw = np.frombuffer(recs[k + 6 + 2 * n_part], dtype="<f8")
+    n_atoms = math.isqrt(len(raw))
+    if n_atoms * n_atoms != len(raw):
+        raise ValueError(f"Net_Charge record holds {len(raw)} values, not n_atoms**2")
+    out["net_charge"] = raw[len(raw) - n_atoms:].copy()
     return out
 
 

@@ -201,6 +201,71 @@ def tio2_slab_adsorbate(seed: int = 2, n_tio2: int = 236) -> Structure:
     return s
 
 
+def _random_rotation(rng) -> np.ndarray:
+    """Uniform random rotation matrix (QR of a Gaussian matrix, sign-fixed, det +1)."""
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def tio2_cluster(seed: int = 6, n_tio2: int = 28, n_c: int = 6, n_n: int = 2, n_h: int = 6,
+                 sigma: float = 0.05) -> Structure:
+    """Small anatase TiO2 cluster in a random orientation with a C/N/H adsorbate layer.
+
+    n_tio2 Ti nearest the block centre and the 2*n_tio2 O nearest them, rotated
+    by a random rotation and jittered by sigma (Angstrom), so Ti-O, Ti-Ti and
+    O-O pairs lie along general directions; C and N atoms (>= 1.35 A apart)
+    sit 1.9-3.5 A above the cluster surface and n_h of them are capped by H.
+    N = 17*n_tio2 + 4*(n_c + n_n) + n_h (28 units: 476 + 32 + 6 = 514).
+    Fragments: 0 = Ti, 1 = O, 2 = C/N/H.
+    """
+    rng = np.random.default_rng(seed)
+    ti, o = _anatase_block(4, 4, 3)
+    center = ti.mean(axis=0)
+    sel_ti = ti[np.argsort(np.linalg.norm(ti - center, axis=1), kind="stable")[:n_tio2]]
+    d_o = np.min(np.linalg.norm(o[:, None, :] - sel_ti[None, :, :], axis=2), axis=1)
+    d_o = d_o + 1e-3 * np.linalg.norm(o - center, axis=1)
+    sel_o = o[np.argsort(d_o, kind="stable")[:2 * n_tio2]]
+    core = np.vstack([sel_ti, sel_o]) - center
+    core = core @ _random_rotation(rng).T
+    core += rng.normal(0.0, sigma, core.shape)
+    # adsorbate atoms on the cluster surface, at general positions
+    heavy = []
+    rmax = np.linalg.norm(core, axis=1).max()
+    while len(heavy) < n_c + n_n:
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        p = v * rng.uniform(0.5 * rmax, rmax + 3.0)
+        dc = np.linalg.norm(core - p, axis=1).min()
+        if not 1.9 <= dc <= 3.5:
+            continue
+        if heavy and np.linalg.norm(np.asarray(heavy) - p, axis=1).min() < 1.35:
+            continue
+        heavy.append(p)
+    heavy = np.asarray(heavy)
+    hs = []
+    for i in range(n_h):
+        for _ in range(100000):
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            q = heavy[i] + 1.09 * v
+            d = np.linalg.norm(np.vstack([core, np.delete(heavy, i, axis=0)]) - q, axis=1).min()
+            if d < 1.5:
+                continue
+            if hs and np.linalg.norm(np.asarray(hs) - q, axis=1).min() < 1.4:
+                continue
+            hs.append(q)
+            break
+        else:
+            raise RuntimeError("could not cap the adsorbate")
+    symbols = ["Ti"] * n_tio2 + ["O"] * (2 * n_tio2) + ["C"] * n_c + ["N"] * n_n + ["H"] * n_h
+    coords = np.vstack([core, heavy, np.asarray(hs).reshape(-1, 3)])
+    frag = np.array([0] * n_tio2 + [1] * (2 * n_tio2) + [2] * (n_c + n_n + n_h), dtype=np.int32)
+    return Structure(symbols, coords, frag, ["Ti", "O", "A"], name=f"TiO2_{n_tio2}_cluster")
+
+
 def interface_cluster(seed: int = 4, n_heavy: int = 1931, n_h: int = 468) -> Structure:
     """C/H/O interface cluster with N = 4*1931 + 468 = 8192 orbitals (2399 atoms).
 

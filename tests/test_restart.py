@@ -45,7 +45,7 @@ def test_security_copy_layout_and_round_trip(tmp_path):
     path = tmp_path / "Security_copy.dat"
     RS.write_security_copy(str(path), *arrs, t=0.125, it=7, net_charge=q, frame=3)
     raw = path.read_bytes()
-    payloads = [4, 4, 8, 4, 4, 4, 4, N * 32, N * 32, N * 32, N * 32, 5 * 8]
+    payloads = [4, 4, 8, 4, 4, 4, 4, N * 32, N * 32, N * 32, N * 32, 5 * 5 * 8]
     assert len(raw) == sum(8 + p for p in payloads)
     # frame, it and t records, framed by their lengths
     assert raw[:12] == struct.pack("<iii", 4, 3, 4)
@@ -72,3 +72,61 @@ def test_net_charge():
     ket = np.array([[0.6, 1.0], [0.8, 0.0], [0.0, 1.0]], dtype=complex)
     q = RS.net_charge(bra, ket, atom, 2)
     assert np.allclose(q, [-(0.36 + 0.64) + 0.5, 0.5])
+
+
+def _fortran_record(items):
+    """One unformatted sequential record: the I/O list's values in order, framed by
+    4-byte little-endian lengths."""
+    body = b"".join(items)
+    return struct.pack("<i", len(body)) + body + struct.pack("<i", len(body))
+
+
+def test_security_copy_bytes_from_io_lists(tmp_path):
+    """The whole file assembled item by item from Security_Copy_Cheb's I/O lists
+    (backup.f:372-386): `( DUAL_bra(i,j) , DUAL_ket(i,j) , i=1,basis_size )` puts
+    the two arrays out interleaved, and `( Net_Charge , i=1,size(Net_Charge) )`
+    puts the whole Net_Charge array out once per i, n_atoms**2 reals."""
+    rng = np.random.default_rng(5)
+    N, n_part, n_atoms = 4, 2, 3
+    db, dk, ab, ak = (rng.normal(size=(N, n_part)) + 1j * rng.normal(size=(N, n_part)) for _ in range(4))
+    q = rng.normal(size=n_atoms)
+    c16 = lambda z: struct.pack("<dd", z.real, z.imag)
+    exp = _fortran_record([struct.pack("<i", 9)])
+    exp += _fortran_record([struct.pack("<d", 0.0625)])
+    exp += _fortran_record([struct.pack("<i", N)]) + _fortran_record([struct.pack("<i", n_part)])
+    exp += _fortran_record([struct.pack("<i", 2)]) + _fortran_record([b"el", b"hl"])
+    for j in range(n_part):
+        exp += _fortran_record([c16(x) for i in range(N) for x in (db[i, j], dk[i, j])])
+        exp += _fortran_record([c16(x) for i in range(N) for x in (ab[i, j], ak[i, j])])
+    exp += _fortran_record([struct.pack("<d", v) for _i in range(n_atoms) for v in q])
+    path = tmp_path / "Security_copy.dat"
+    RS.write_security_copy(str(path), db, dk, ab, ak, t=0.0625, it=9, net_charge=q)
+    assert path.read_bytes() == exp
+
+
+def test_restart_reads_reference_layout_file(tmp_path):
+    """A Restart_copy.dat written in the reference's layout (Net_Charge repeated
+    n_atoms times; the copies need not agree) reads back as Restart_State_Cheb
+    reads it: the last copy stays in Net_Charge (backup.f:436)."""
+    N, n_atoms = 2, 3
+    z = np.array([[1 + 2j], [3 - 1j]])
+    copies = [np.array([0.1, 0.2, 0.3]), np.array([0.4, 0.5, 0.6]), np.array([-1.0, 0.25, 2.0])]
+    c16 = lambda v: struct.pack("<dd", v.real, v.imag)
+    raw = _fortran_record([struct.pack("<i", 4)]) + _fortran_record([struct.pack("<d", 1.5)])
+    raw += _fortran_record([struct.pack("<i", N)]) + _fortran_record([struct.pack("<i", 1)])
+    raw += _fortran_record([struct.pack("<i", 1)]) + _fortran_record([b"el"])
+    raw += _fortran_record([c16(x) for i in range(N) for x in (z[i, 0], -z[i, 0])])
+    raw += _fortran_record([c16(x) for i in range(N) for x in (2 * z[i, 0], 3 * z[i, 0])])
+    raw += _fortran_record([struct.pack("<d", v) for c in copies for v in c])
+    path = tmp_path / "Restart_copy.dat"
+    path.write_bytes(raw)
+    back = RS.read_restart_copy(str(path))
+    assert back["it"] == 4 and back["t"] == 1.5 and back["eh_tag"] == ("el",)
+    assert np.array_equal(back["dual_bra"], z) and np.array_equal(back["dual_ket"], -z)
+    assert np.array_equal(back["ao_bra"], 2 * z) and np.array_equal(back["ao_ket"], 3 * z)
+    assert np.array_equal(back["net_charge"], copies[-1])
+    # and what this library writes is read back the same way
+    RS.write_security_copy(str(tmp_path / "again.dat"), back["dual_bra"], back["dual_ket"], back["ao_bra"],
+                           back["ao_ket"], t=1.5, it=4, net_charge=back["net_charge"], eh_tag=("el",))
+    again = RS.read_restart_copy(str(tmp_path / "again.dat"))
+    assert np.array_equal(again["net_charge"], copies[-1])
