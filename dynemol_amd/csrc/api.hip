@@ -785,50 +785,64 @@ static int ensure_coefficients(dh_system* s, double dt) {
     return DH_OK;
 }
 
-// One Chebyshev evaluation phi <- sum_k a_k T_k(Hh) phi, result in s->acc.
-static int series_launches(dh_system* s) {
-    hipStream_t st = s->ctx->stream;
-    const int R = 2 * s->W;
-    const double c = s->coef_c, r = s->coef_r;
+// Operands of one Chebyshev evaluation acc <- sum_k a_k T_k(Hh) phi, Hh = (H - c)/r,
+// with the device coefficients a_0..a_K already in coef.  phi is read only;
+// the recurrence alternates between x0 and x1.
+struct SeriesOps {
+    const double* H;
+    const float* H32;      // precision modes 1 and 2
+    int ld, N, R;
+    const cplx* phi;
+    cplx *x0, *x1, *acc, *partial;
+    unsigned* counters;
+    const cplx* coef;
+    float* x32[2];         // precision mode 1
+    int prec, K;
+    double c, r;
+};
+
+static void series_core(const SeriesOps& o, hipStream_t st) {
+    const int R = o.R;
+    const double c = o.c, r = o.r;
     MatvecArgs a{};
-    a.M = s->Hth;
-    a.ld = s->ld;
-    a.N = s->N;
+    a.M = o.H;
+    a.ld = o.ld;
+    a.N = o.N;
     a.R = R;
-    a.partial = s->partial;
-    a.counters = s->counters;
-    a.split = matvec_split_for(s->N, R);
+    a.partial = o.partial;
+    a.counters = o.counters;
+    a.split = matvec_split_for(o.N, R);
     a.tri = TRI_FULL;
     a.x_stride = R; a.x_off = 0;
     a.out_stride = R; a.out_off = 0;
-    a.acc = s->acc;
-    a.coef = s->coef;
-    a.M32 = s->H32;
-    const int prec = s->precision;
+    a.acc = o.acc;
+    a.coef = o.coef;
+    a.M32 = o.H32;
+    const int prec = o.prec;
     // mode 1: the FP32 GEMM reads FP32 copies of its input; the step's state
     // is converted once, then each term's epilogue writes the next one's
-    if (prec == 1) launch_to_f32(reinterpret_cast<const double*>(s->phi), s->x32[0], (long)s->ld * 2 * R, st);
+    if (prec == 1) launch_to_f32(reinterpret_cast<const double*>(o.phi), o.x32[0], (long)o.ld * 2 * R, st);
     auto term = [&](MatvecArgs args) {
         if (prec == 1) {
-            args.x32 = s->x32[(args.k - 1) & 1];
-            args.out32 = s->x32[args.k & 1];
+            args.x32 = o.x32[(args.k - 1) & 1];
+            args.out32 = o.x32[args.k & 1];
         }
-        if (prec == 1) launch_gemm3(args, 1, reinterpret_cast<double*>(s->partial), st);
+        if (prec == 1) launch_gemm3(args, 1, reinterpret_cast<double*>(o.partial), st);
         else if (prec == 2) launch_term_f32h(args, st);
         else launch_matvec(args, st);
     };
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
     a.epi = EPI_CHEB_FIRST;
-    a.x = s->phi;
-    a.out = s->x1;
+    a.x = o.phi;
+    a.out = o.x1;
     a.alpha = 1.0 / r;
     a.beta = -c / r;
     a.k = 1;
     term(a);
-    cplx* prev2 = s->phi;
-    cplx* prev = s->x1;
-    for (int k = 2; k <= s->K; ++k) {
-        cplx* outb = (k == 2) ? s->x0 : prev2;
+    const cplx* prev2 = o.phi;
+    cplx* prev = o.x1;
+    for (int k = 2; k <= o.K; ++k) {
+        cplx* outb = (k == 2) ? o.x0 : const_cast<cplx*>(prev2);
         a.epi = EPI_CHEB;
         a.x = prev;
         a.x2 = prev2;
@@ -840,6 +854,30 @@ static int series_launches(dh_system* s) {
         prev2 = prev;
         prev = outb;
     }
+}
+
+// One Chebyshev evaluation phi <- sum_k a_k T_k(Hh) phi, result in s->acc.
+static int series_launches(dh_system* s) {
+    SeriesOps o{};
+    o.H = s->Hth;
+    o.H32 = s->H32;
+    o.ld = s->ld;
+    o.N = s->N;
+    o.R = 2 * s->W;
+    o.phi = s->phi;
+    o.x0 = s->x0;
+    o.x1 = s->x1;
+    o.acc = s->acc;
+    o.partial = s->partial;
+    o.counters = s->counters;
+    o.coef = s->coef;
+    o.x32[0] = s->x32[0];
+    o.x32[1] = s->x32[1];
+    o.prec = s->precision;
+    o.K = s->K;
+    o.c = s->coef_c;
+    o.r = s->coef_r;
+    series_core(o, s->ctx->stream);
     DH_HIP(hipGetLastError());
     return DH_OK;
 }
@@ -1064,8 +1102,179 @@ extern "C" int dh_propagation_elhl(dh_context* ctx, const int* Np, const double*
     return DH_OK;
 }
 
+// ------------------------------------------------------------------ caller-owned device operands
+// Work layout of dh_chebyshev_apply: recurrence vectors, the term kernels'
+// partial sums (enough for every variant the autotuner may pick for (N, R)),
+// counters, Bessel table and coefficients, norms; 256-byte aligned pieces.
+struct ApplyWork {
+    cplx *x0, *x1, *acc, *partial;
+    unsigned* counters;
+    double *J, *norms0, *norms1;
+    cplx* coef;
+    int* kout;
+};
+
+static size_t apply_layout(int N, int R, char* base, ApplyWork* w) {
+    const int ld = round_ld(N);
+    long part = (long)matvec_split_for(N, R) * ld * R;  // split-K column-block variants
+    if (sym_supported(R, TRI_FULL)) part = std::max(part, std::max(sym_partial_cplx(ld, R), sym2_partial_cplx(ld, R)));
+    part = std::max(part, (gemm3_partial_doubles(N, ld, R) + 1) / 2);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    ApplyWork t{};
+    t.x0 = (cplx*)take(sizeof(cplx) * ld * R);
+    t.x1 = (cplx*)take(sizeof(cplx) * ld * R);
+    t.acc = (cplx*)take(sizeof(cplx) * ld * R);
+    t.partial = (cplx*)take(sizeof(cplx) * part);
+    t.counters = (unsigned*)take(sizeof(unsigned) * ld);
+    t.J = (double*)take(sizeof(double) * (KMAX_ALLOC + 2));
+    t.coef = (cplx*)take(sizeof(cplx) * (KMAX_ALLOC + 2));
+    t.kout = (int*)take(sizeof(int) * 4);
+    t.norms0 = (double*)take(sizeof(double) * R);
+    t.norms1 = (double*)take(sizeof(double) * R);
+    if (w) *w = t;
+    return off;
+}
+
+extern "C" long long dh_chebyshev_work_bytes(int N, int R) {
+    if (N <= 0 || !(R == 1 || R == 2 || R == 4 || R == 8)) return -1;
+    return (long long)apply_layout(N, R, nullptr, nullptr);
+}
+
+// phi <- exp(-i H dt / hbar) phi on the context stream, with the work buffer
+// carved as apply_layout gives it.  Shared by the workspace and the
+// allocation-free entry points, so both run the same kernels on the same shape.
+static int chebyshev_apply_impl(dh_context* ctx, int N, const double* H, int ldh, const double* bounds, double dt_ps,
+                                int R, double* phi_d, char* work, int* n_terms) {
+    hipStream_t st = ctx->stream;
+    ApplyWork w{};
+    apply_layout(N, R, work, &w);
+    cplx* phi = reinterpret_cast<cplx*>(phi_d);
+    // rows N..ld-1 of the state are padding: zero, so they stay zero under H
+    if (ldh > N) launch_zero_cplx(phi + (size_t)N * R, (long)(ldh - N) * R, st);
+    launch_zero_u32(w.counters, ldh, st);
+    const double dt = dt_ps / H_BAR;
+    if (!(dt > 0.0)) {
+        if (n_terms) *n_terms = 0;
+        DH_HIP(hipGetLastError());
+        DH_HIP(hipStreamSynchronize(st));
+        return DH_OK;
+    }
+    launch_column_norms(N, R, phi, w.norms0, st);
+    SeriesOps o{};
+    o.H = H;
+    o.ld = ldh;
+    o.N = N;
+    o.R = R;
+    o.phi = phi;
+    o.x0 = w.x0;
+    o.x1 = w.x1;
+    o.acc = w.acc;
+    o.partial = w.partial;
+    o.counters = w.counters;
+    o.coef = w.coef;
+    o.prec = 0;
+    double widen = 1.0, drift = 0.0;
+    std::vector<double> n0(R), n1(R);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        const double c = 0.5 * (bounds[1] + bounds[0]);
+        const double r = 0.5 * (bounds[1] - bounds[0]) * widen;
+        launch_bessel(r * dt, c * dt, KMAX_ALLOC, 1e-18, w.J, w.coef, w.kout, st);
+        int K = 0;
+        DH_HIP(hipGetLastError());
+        DH_HIP(hipMemcpyAsync(&K, w.kout, sizeof(int), hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        if (K <= 0 && r * dt > 1e-12) {
+            dh::set_error("dh_chebyshev_apply: no Chebyshev terms for r*dt = %g", r * dt);
+            return DH_ERR_STATE;
+        }
+        o.K = K;
+        o.c = c;
+        o.r = r;
+        series_core(o, st);
+        launch_column_norms(N, R, w.acc, w.norms1, st);
+        DH_HIP(hipGetLastError());
+        DH_HIP(hipMemcpyAsync(n0.data(), w.norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(n1.data(), w.norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        drift = 0.0;
+        for (int col = 0; col < R; ++col) {
+            if (n0[col] > 0) drift = std::max(drift, std::fabs(n1[col] - n0[col]) / n0[col]);
+            if (!std::isfinite(n1[col])) drift = INFINITY;
+        }
+        if (drift < 1e-9) {
+            DH_HIP(hipMemcpyAsync(phi, w.acc, sizeof(cplx) * (size_t)N * R, hipMemcpyDeviceToDevice, st));
+            DH_HIP(hipStreamSynchronize(st));
+            if (n_terms) *n_terms = K + 1;
+            return DH_OK;
+        }
+        widen *= 1.25;  // the bounds did not enclose the spectrum; phi is untouched
+    }
+    dh::set_error("dh_chebyshev_apply: Chebyshev series lost the norm (drift %g) after widening the bounds", drift);
+    return DH_ERR_DIVERGED;
+}
+
+static int check_apply_args(const char* who, dh_context* ctx, int N, const double* H, int ldh, const double* bounds,
+                            int R, const double* phi) {
+    if (!(ctx && H && bounds && phi)) {
+        dh::set_error("%s: null argument", who);
+        return DH_ERR_ARG;
+    }
+    if (N <= 0 || ldh < N || ldh % TILE != 0 || ldh != round_ld(N)) {
+        dh::set_error("%s: need N > 0 and ldh = N rounded up to a multiple of %d (got N = %d, ldh = %d)", who, TILE, N,
+                      ldh);
+        return DH_ERR_ARG;
+    }
+    if (!(R == 1 || R == 2 || R == 4 || R == 8)) {
+        dh::set_error("%s: R must be 1, 2, 4 or 8 (got %d)", who, R);
+        return DH_ERR_ARG;
+    }
+    if (!(bounds[1] > bounds[0]) || !std::isfinite(bounds[0]) || !std::isfinite(bounds[1])) {
+        dh::set_error("%s: bounds must be finite with E_max > E_min", who);
+        return DH_ERR_ARG;
+    }
+    return DH_OK;
+}
+
+extern "C" int dh_chebyshev_apply(dh_context* ctx, int N, const double* H, int ldh, const double* bounds, double dt,
+                                  int R, double* phi, void* work, long long work_bytes, int* n_terms) {
+    int rc = check_apply_args("dh_chebyshev_apply", ctx, N, H, ldh, bounds, R, phi);
+    if (rc) return rc;
+    const long long need = dh_chebyshev_work_bytes(N, R);
+    if (!work || work_bytes < need || (reinterpret_cast<uintptr_t>(work) & 255)) {
+        dh::set_error("dh_chebyshev_apply: work must be a 256-byte aligned device buffer of at least %lld bytes "
+                      "(dh_chebyshev_work_bytes)", need);
+        return DH_ERR_ARG;
+    }
+    DH_HIP(hipSetDevice(ctx->device));
+    return chebyshev_apply_impl(ctx, N, H, ldh, bounds, dt, R, phi, static_cast<char*>(work), n_terms);
+}
+
 // ------------------------------------------------------------------ test hooks
 #include "../../include/dynemol_hip_testing.h"
+
+extern "C" int dh_test_device_alloc(long long bytes, void** ptr) {
+    CHECK_ARG(bytes > 0 && ptr, "dh_test_device_alloc: bad argument");
+    char* p = nullptr;
+    DALLOC(p, (size_t)bytes);
+    *ptr = p;
+    return DH_OK;
+}
+
+extern "C" int dh_test_device_free(void* ptr) {
+    DH_HIP(hipFree(ptr));
+    return DH_OK;
+}
+
+extern "C" int dh_test_memcpy(void* dst, const void* src, long long bytes, int to_device) {
+    CHECK_ARG(dst && src && bytes >= 0, "dh_test_memcpy: bad argument");
+    DH_HIP(hipMemcpy(dst, src, (size_t)bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+    return DH_OK;
+}
 
 extern "C" int dh_test_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
                             int ldc, double alpha, double beta, int transA, int transB, int kmode, int lower_only) {

@@ -36,9 +36,11 @@ SYMBOLS = [
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
     "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
+    "dh_chebyshev_work_bytes", "dh_chebyshev_apply",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
-                "dh_test_matvec_timing", "dh_test_read"]
+                "dh_test_matvec_timing", "dh_test_read", "dh_test_device_alloc", "dh_test_device_free",
+                "dh_test_memcpy"]
 
 # Every live context and system is released before interpreter shutdown, while
 # the HIP runtime is still up (a handle freed during finalisation aborts).
@@ -107,6 +109,13 @@ def lib():
     L.dh_batch_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
     L.dh_test_force_matvec_variant.argtypes = [ctypes.c_int]
     L.dh_test_matvec_timing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
+    L.dh_chebyshev_work_bytes.restype = ctypes.c_longlong
+    L.dh_chebyshev_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.dh_chebyshev_apply.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _d, ctypes.c_double, ctypes.c_int, _vp,
+                                     _vp, ctypes.c_longlong, _i]
+    L.dh_test_device_alloc.argtypes = [ctypes.c_longlong, ctypes.POINTER(_vp)]
+    L.dh_test_device_free.argtypes = [_vp]
+    L.dh_test_memcpy.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_int]
     _lib = L
     return L
 
@@ -400,3 +409,45 @@ def run_copy(nbytes, repeats=20):
 
 def system_bytes(n_orb, n_wp):
     return int(lib().dh_system_bytes(n_orb, n_wp))
+
+
+def chebyshev_work_bytes(N, R):
+    return int(lib().dh_chebyshev_work_bytes(int(N), int(R)))
+
+
+def chebyshev_apply(ctx: Context, N, H_dev, ldh, bounds, dt, R, phi_dev, work_dev, work_bytes):
+    """phi <- exp(-i H dt / hbar) phi on caller-owned device buffers (device
+    addresses as ints); returns the Chebyshev term count."""
+    b = np.ascontiguousarray(bounds[:2], dtype=np.float64)
+    n = ctypes.c_int(0)
+    _check(lib().dh_chebyshev_apply(ctx.h, int(N), _vp(H_dev), int(ldh), _p(b), float(dt), int(R), _vp(phi_dev),
+                                    _vp(work_dev), int(work_bytes), ctypes.byref(n)), "dh_chebyshev_apply")
+    return n.value
+
+
+class DeviceBuffer:
+    """Device memory through the library's test hooks (for the entry points that
+    take device pointers)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(lib().dh_test_device_alloc(self.nbytes, ctypes.byref(p)), "dh_test_device_alloc")
+        self.ptr = p.value
+        _LIVE.add(self)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().dh_test_memcpy(_vp(self.ptr), a.ctypes.data_as(_vp), a.nbytes, 1), "dh_test_memcpy")
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().dh_test_memcpy(out.ctypes.data_as(_vp), _vp(self.ptr), out.nbytes, 0), "dh_test_memcpy")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().dh_test_device_free(_vp(self.ptr))
+            self.ptr = None

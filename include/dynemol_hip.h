@@ -194,6 +194,24 @@ int dh_propagation_elhl(dh_context *ctx, const int *N, const double *S, const do
                         double *AO_bra, double *AO_ket, double *PSI_bra, double *PSI_ket, const double *t_init,
                         const double *t_max, double *tau, double *save_tau);
 
+/*
+ * Propagation of caller-owned device operands: phi <- exp(-i H dt / hbar) phi
+ * for R (1, 2, 4 or 8) complex columns, by the Chebyshev series on
+ * [bounds[0], bounds[1]] (eV; e.g. dh_get_bounds) with the same term kernels
+ * and norm check as dh_propagate (the role of Propagation, Taylor.f:35-127, on
+ * an already orthogonalised operator; Taylor_gpu.cpp:295-330 applies H' the
+ * same way once per term).  H: device, ldh x ldh column-major, symmetric, ldh =
+ * N rounded up to a multiple of 64, rows and columns N..ldh-1 zero off the
+ * diagonal (as H_orth is stored).  phi: device, ldh rows x R complex, row-major
+ * (column c of row i at phi[2 (i R + c)]); rows N..ldh-1 are set to zero.
+ * dt in ps.  work: a 256-byte aligned device buffer of at least
+ * dh_chebyshev_work_bytes(N, R) bytes.  Runs on the context stream and returns
+ * when phi holds the result; n_terms (may be NULL) receives the term count.
+ */
+long long dh_chebyshev_work_bytes(int N, int R);
+int dh_chebyshev_apply(dh_context *ctx, int N, const double *H, int ldh, const double *bounds, double dt, int R,
+                       double *phi, void *work, long long work_bytes, int *n_terms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,11 @@ int dh_test_read(long long bytes, int repeats, double *ms_per_read);
 int dh_test_matvec_timing(int N, int R, int variant, int reps, double *us_per_launch);
 /* Force every matvec to one kernel variant (-1 restores the autotuned choice). */
 int dh_test_force_matvec_variant(int variant);
+/* Device memory for tests of the entry points that take device pointers:
+ * allocate, free, and copy to (to_device = 1) or from (0) the device. */
+int dh_test_device_alloc(long long bytes, void **ptr);
+int dh_test_device_free(void *ptr);
+int dh_test_memcpy(void *dst, const void *src, long long bytes, int to_device);
 #ifdef __cplusplus
 }
 #endif
