@@ -38,6 +38,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -809,6 +810,10 @@ int orc_propagation(int n, const double *Hp, double *bra_io, double *ket_io, dou
     free(C);
     return guard > ORC_GUARD ? -1 : 0;
 }
+
+/* OpenMP thread count of the following calls (the CPU baseline runs all cores and one). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 /* Bare GEMV kernels of the Taylor path, exported for the CPU baseline. */
 void orc_op_x_ket(int n, const double *H, const double *ket, double *res) {

@@ -56,6 +56,8 @@ def lib():
         L.orc_propagation.argtypes = [ctypes.c_int, _d, _d, _d, ctypes.c_double, ctypes.c_double, _d, _d,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _l]
         L.orc_op_x_ket.argtypes = [ctypes.c_int, _d, _d, _d]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_max_threads.restype = ctypes.c_int
         L.orc_bra_x_op.argtypes = [ctypes.c_int, _d, _d, _d]
         L.orc_populations.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d]
         L.orc_qp_energy.argtypes = [ctypes.c_int, _d, _d, _d, _d]
@@ -197,6 +199,24 @@ def propagation(Hp, bra, ket, t_init, t_max, tau, err_tol=1e-8, norm_tol=1e-8, o
         raise RuntimeError("Taylor propagation did not converge")
     return bra, ket, float(t[0]), float(st[0]), {"matvec": int(stats[0]), "substeps": int(stats[1]),
                                                   "rescale": int(stats[2]), "shrink": int(stats[3])}
+
+
+def set_threads(n):
+    """OpenMP threads of the following oracle calls."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
+def op_x_ket(Hp, ket):
+    """One Taylor-path GEMV, H' ket (the dzgemv of Matrix_math.f:225-301)."""
+    Hp = np.asfortranarray(Hp)
+    ket = np.ascontiguousarray(ket, dtype=np.complex128)
+    out = np.zeros_like(ket)
+    lib().orc_op_x_ket(Hp.shape[0], Hp.ctypes.data_as(_d), ket.ctypes.data_as(_d), out.ctypes.data_as(_d))
+    return out
 
 
 def populations(frag, n_frag, dual_bra, dual_ket):
