@@ -32,7 +32,7 @@ def ctx():
 
 def _operator(ctx, st):
     b = st.basis()
-    s = hip.System(ctx, b, n_wp=1, n_frag=1)
+    s = hip.System(ctx, b, n_wp=1, n_frag=len(st.fragment_names))
     s.build(st.coords)
     Ho = s.matrix(hip.DH_MAT_HORTH)
     bounds = s.bounds()[:2].copy()
@@ -89,7 +89,7 @@ def test_apply_equals_system_step(ctx):
     st = tio2_cluster(seed=6)
     b = st.basis()
     N = b.N
-    s = hip.System(ctx, b, n_wp=2, n_frag=1)
+    s = hip.System(ctx, b, n_wp=2, n_frag=len(st.fragment_names))
     s.build(st.coords)
     Ho = s.matrix(hip.DH_MAT_HORTH)
     L = np.tril(s.matrix(hip.DH_MAT_L))
