@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dynemol_hip.h"
@@ -27,6 +28,35 @@ using namespace dh;
 static const double H_BAR = 6.58264e-4;  // eV*ps, constants_m.f:23
 static const int KMAX_ALLOC = 4096;      // Chebyshev terms supported (r*dt up to ~3500)
 static const int LANCZOS_MAX = 120;
+
+// Scratch of dh_chebyshev_apply_noalloc: one copy per device, held until the
+// process ends.  12 MiB covers the work layout of N <= 4096 with R <= 4 and of
+// N <= 2048 with R <= 8.  Host code never uses the array's name as an address:
+// dh_init looks up each device's copy with hipGetSymbolAddress and keeps it in
+// g_scratch_dev.
+static constexpr long long NOALLOC_SCRATCH_BYTES = 12LL << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int MAX_DEVICES = 64;
+static void* g_scratch_dev[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (once per device).
+static int scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) {
+        dh::set_error("dh_init: device index %d beyond %d", device, MAX_DEVICES);
+        return DH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return DH_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        dh::set_error("dh_init: hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        return DH_ERR_HIP;
+    }
+    g_scratch_dev[device] = ptr;
+    return DH_OK;
+}
 
 struct dh_context {
     int device = 0;
@@ -138,6 +168,8 @@ extern "C" int dh_init(int device, dh_context** out) {
     DH_HIP(hipGetDeviceCount(&n));
     CHECK_ARG(device >= 0 && device < n, "dh_init: device index out of range");
     DH_HIP(hipSetDevice(device));
+    const int src = scratch_setup(device);
+    if (src) return src;
     dh_context* c = new dh_context();
     c->device = device;
     DH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1179,7 +1211,7 @@ static int chebyshev_apply_impl(dh_context* ctx, int N, const double* H, int ldh
     o.coef = w.coef;
     o.prec = 0;
     double widen = 1.0, drift = 0.0;
-    std::vector<double> n0(R), n1(R);
+    double n0[8], n1[8];  // R <= 8 (check_apply_args); no host allocation on this path
     for (int attempt = 0; attempt < 4; ++attempt) {
         const double c = 0.5 * (bounds[1] + bounds[0]);
         const double r = 0.5 * (bounds[1] - bounds[0]) * widen;
@@ -1198,8 +1230,8 @@ static int chebyshev_apply_impl(dh_context* ctx, int N, const double* H, int ldh
         series_core(o, st);
         launch_column_norms(N, R, w.acc, w.norms1, st);
         DH_HIP(hipGetLastError());
-        DH_HIP(hipMemcpyAsync(n0.data(), w.norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
-        DH_HIP(hipMemcpyAsync(n1.data(), w.norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(n0, w.norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(n1, w.norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         DH_HIP(hipStreamSynchronize(st));
         drift = 0.0;
         for (int col = 0; col < R; ++col) {
@@ -1252,6 +1284,26 @@ extern "C" int dh_chebyshev_apply(dh_context* ctx, int N, const double* H, int l
     }
     DH_HIP(hipSetDevice(ctx->device));
     return chebyshev_apply_impl(ctx, N, H, ldh, bounds, dt, R, phi, static_cast<char*>(work), n_terms);
+}
+
+extern "C" int dh_chebyshev_apply_noalloc(dh_context* ctx, int N, const double* H, int ldh, const double* bounds,
+                                          double dt, int R, double* phi, int* n_terms) {
+    int rc = check_apply_args("dh_chebyshev_apply_noalloc", ctx, N, H, ldh, bounds, R, phi);
+    if (rc) return rc;
+    const long long need = dh_chebyshev_work_bytes(N, R);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        dh::set_error("dh_chebyshev_apply_noalloc: N = %d, R = %d needs %lld bytes of scratch, beyond the %lld the "
+                      "library holds (N <= 4096 with R <= 4, or N <= 2048 with R <= 8); use dh_chebyshev_apply",
+                      N, R, need, NOALLOC_SCRATCH_BYTES);
+        return DH_ERR_ARG;
+    }
+    void* ptr = (ctx->device >= 0 && ctx->device < MAX_DEVICES) ? g_scratch_dev[ctx->device] : nullptr;
+    if (!ptr) {
+        dh::set_error("dh_chebyshev_apply_noalloc: no scratch address for device %d", ctx->device);
+        return DH_ERR_STATE;
+    }
+    DH_HIP(hipSetDevice(ctx->device));
+    return chebyshev_apply_impl(ctx, N, H, ldh, bounds, dt, R, phi, static_cast<char*>(ptr), n_terms);
 }
 
 // ------------------------------------------------------------------ test hooks

@@ -36,7 +36,7 @@ SYMBOLS = [
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
     "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
-    "dh_chebyshev_work_bytes", "dh_chebyshev_apply",
+    "dh_chebyshev_work_bytes", "dh_chebyshev_apply", "dh_chebyshev_apply_noalloc",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read", "dh_test_device_alloc", "dh_test_device_free",
@@ -113,6 +113,8 @@ def lib():
     L.dh_chebyshev_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
     L.dh_chebyshev_apply.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _d, ctypes.c_double, ctypes.c_int, _vp,
                                      _vp, ctypes.c_longlong, _i]
+    L.dh_chebyshev_apply_noalloc.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _d, ctypes.c_double,
+                                             ctypes.c_int, _vp, _i]
     L.dh_test_device_alloc.argtypes = [ctypes.c_longlong, ctypes.POINTER(_vp)]
     L.dh_test_device_free.argtypes = [_vp]
     L.dh_test_memcpy.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_int]
@@ -422,6 +424,15 @@ def chebyshev_apply(ctx: Context, N, H_dev, ldh, bounds, dt, R, phi_dev, work_de
     n = ctypes.c_int(0)
     _check(lib().dh_chebyshev_apply(ctx.h, int(N), _vp(H_dev), int(ldh), _p(b), float(dt), int(R), _vp(phi_dev),
                                     _vp(work_dev), int(work_bytes), ctypes.byref(n)), "dh_chebyshev_apply")
+    return n.value
+
+
+def chebyshev_apply_noalloc(ctx: Context, N, H_dev, ldh, bounds, dt, R, phi_dev):
+    """chebyshev_apply with the library's static scratch instead of a work buffer."""
+    b = np.ascontiguousarray(bounds[:2], dtype=np.float64)
+    n = ctypes.c_int(0)
+    _check(lib().dh_chebyshev_apply_noalloc(ctx.h, int(N), _vp(H_dev), int(ldh), _p(b), float(dt), int(R),
+                                            _vp(phi_dev), ctypes.byref(n)), "dh_chebyshev_apply_noalloc")
     return n.value
 
 

@@ -211,6 +211,19 @@ int dh_propagation_elhl(dh_context *ctx, const int *N, const double *S, const do
 long long dh_chebyshev_work_bytes(int N, int R);
 int dh_chebyshev_apply(dh_context *ctx, int N, const double *H, int ldh, const double *bounds, double dt, int R,
                        double *phi, void *work, long long work_bytes, int *n_terms);
+/*
+ * dh_chebyshev_apply without the work argument and without any device
+ * allocation: for callers that cannot allocate (code under stream capture set
+ * up beforehand, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The work buffer is a static 12 MiB device
+ * array, one per GPU, held until the process ends: N <= 4096 with R <= 4, or
+ * N <= 2048 with R <= 8 (dh_chebyshev_work_bytes(N, R) <= 12 MiB); larger
+ * shapes are refused with DH_ERR_ARG.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (from several contexts or threads)
+ * race.  Same kernels and results as dh_chebyshev_apply.
+ */
+int dh_chebyshev_apply_noalloc(dh_context *ctx, int N, const double *H, int ldh, const double *bounds, double dt,
+                               int R, double *phi, int *n_terms);
 
 #ifdef __cplusplus
 }
