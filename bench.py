@@ -78,47 +78,38 @@ def initial_states(st, b):
 
 
 def run_gpu(args, st, b, S, lumo, homo):
+    """The drop-in's nuclear-step loop on the device (dh_propagate_steps): W
+    warm-up steps, then the timed region: the rebuild at its first step from new
+    coordinates and K steps with a rebuild every REBUILD_EVERY steps, each from
+    coordinates jittered by JITTER (all geometries generated before the clock
+    starts; the library copies them to the device once per call)."""
     from dynemol_amd import hip
     ctx = hip.Context(0)
     sysm = hip.System(ctx, b, n_wp=2, n_frag=len(st.fragment_names))
     rng = np.random.default_rng(2)
-    coords = st.coords.copy()
-    sysm.build(coords)
+    sysm.build(st.coords)
     for w, c in enumerate((lumo, homo)):
         sysm.set_wavepacket(w, S @ c + 0j, c + 0j)
 
-    t = 0.0
-    rebuild_s = 0.0
-    step = 0
+    def geometries(n):
+        return np.stack([st.coords + rng.normal(0.0, JITTER, st.coords.shape) for _ in range(max(n, 0))]) \
+            if n > 0 else None
 
-    def one_step():
-        nonlocal t, step, rebuild_s, coords
-        if step > 0 and step % REBUILD_EVERY == 0:
-            t0 = time.perf_counter()
-            coords = st.coords + rng.normal(0.0, JITTER, st.coords.shape)
-            sysm.build(coords)
-            rebuild_s_local = time.perf_counter() - t0
-        else:
-            rebuild_s_local = 0.0
-        k = sysm.propagate(t, t + DT_PS)
-        t += DT_PS
-        step += 1
-        return k, rebuild_s_local
-
-    for _ in range(args.warmup):
-        one_step()
+    if args.warmup > 0:
+        sysm.propagate_steps(args.warmup, DT_PS, REBUILD_EVERY, geometries((args.warmup - 1) // REBUILD_EVERY))
+    first = st.coords + rng.normal(0.0, JITTER, st.coords.shape)
+    geos = geometries((args.steps - 1) // REBUILD_EVERY)
     sysm.set_timing(True)
     st0 = sysm.stats()
-    terms = []
-    rebuild_s = 0.0
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        k, rb = one_step()
-        terms.append(k)
-        rebuild_s += rb
+    t0 = time.perf_counter()
+    sysm.build(first)
+    first_build_s = time.perf_counter() - t0
+    pops_hist, erg_hist, k = sysm.propagate_steps(args.steps, DT_PS, REBUILD_EVERY, geos)
     elapsed = time.perf_counter() - t_start
     st1 = sysm.stats()
-    pops = [sysm.observables(w)[1] for w in range(2)]
+    rebuild_s = first_build_s + (st1["rebuild_ms_in_steps"] - st0["rebuild_ms_in_steps"]) / 1e3
+    pops = [pops_hist[-1, w] for w in range(2)]
     bounds = sysm.bounds()
     tuning = sysm.tuning()
     ehrenfest = None if args.no_ehrenfest else time_ehrenfest(sysm)
@@ -128,7 +119,7 @@ def run_gpu(args, st, b, S, lumo, homo):
     series_ms = st1["series_ms"] - st0["series_ms"]
     launches = st1["series_launches"] - st0["series_launches"]
     return {
-        "elapsed": elapsed, "rebuild_s": rebuild_s, "terms": terms, "series_ms": series_ms,
+        "elapsed": elapsed, "rebuild_s": rebuild_s, "terms": [k], "series_ms": series_ms,
         "series_launches": launches, "pops": pops, "bounds": bounds.tolist(), "rebuilds": st1["rebuilds"] - st0["rebuilds"],
         "tuning": tuning, "ehrenfest": ehrenfest, "eigenpairs": eigen,
     }
@@ -238,8 +229,13 @@ def run_batched(st, b, S, n_wp, steps, seed=11):
             "mixed_pop_err_vs_fp64": err, "unit": "wavepacket-steps/s"}
 
 
-def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96):
-    """Poisson arrivals; N log-uniform, 1-16 wavepackets, 10-200 steps each (one context)."""
+def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96, check=True):
+    """Poisson arrivals; N log-uniform, 1-16 wavepackets, 10-200 steps each;
+    requests of 8 or more wavepackets run in the mixed precision mode (the
+    multi-RHS FP32 matrix-core term) beside the FP64 ones.  After the stream
+    (untimed), each FP64 request of N <= 2048 is checked against the exact
+    propagator of its device-built S and h, and each mixed request of N <= 4096
+    against the same request run alone in FP64."""
     from dynemol_amd import hip
     from dynemol_amd.serve import Request, Server, latency_stats
     from dynemol_amd.structures import heavy_cluster, log_uniform_int
@@ -254,17 +250,66 @@ def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96):
         ao = [(rng.normal(size=N) + 1j * rng.normal(size=N)) / np.sqrt(2.0 * N) for _ in range(w)]
         t += rng.exponential(1.0 / lam)
         reqs.append(Request(rid=i, basis=b, coords=st.coords, psi_bra=None, psi_ket=ao, ao_init=True,
-                            steps=int(rng.integers(10, 201)), arrival=t, n_frag=2))
-    log(f"serve: {n_req} requests generated, N in [{min(r.basis.N for r in reqs)}, {max(r.basis.N for r in reqs)}]")
-    with hip.Context(0) as ctx:
-        srv = Server(ctx, hbm_budget_bytes=budget_gib * 2**30)
-        t0 = time.perf_counter()
-        done = srv.run(reqs, log=log)
-        el = time.perf_counter() - t0
+                            steps=int(rng.integers(10, 201)), arrival=t, n_frag=2, mixed=w >= 8))
+    log(f"serve: {n_req} requests generated, N in [{min(r.basis.N for r in reqs)}, {max(r.basis.N for r in reqs)}], "
+        f"{sum(r.mixed for r in reqs)} mixed")
+    srv = Server(0, hbm_budget_bytes=budget_gib * 2**30)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, log=log)
+    el = time.perf_counter() - t0
     st = latency_stats(done)
     st.update({"requests": n_req, "arrival_rate_per_s": lam, "throughput_wp_steps_per_s": st["wavepacket_steps"] / el,
-               "wall_s": el})
-    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+               "wall_s": el, "mixed_requests": sum(r.mixed for r in reqs), "lanes": 1 + srv.n_solo_lanes,
+               "max_concurrent": _max_concurrent(done)})
+    if check:
+        st.update(_serve_errors(done))
+    return {k: (round(v, 4) if isinstance(v, float) and v > 1e-3 else v) for k, v in st.items()}
+
+
+def _max_concurrent(done):
+    ev = sorted([(r.admitted, 1) for r in done] + [(r.finished, -1) for r in done])
+    cur = best = 0
+    for _, d in ev:
+        cur += d
+        best = max(best, cur)
+    return best
+
+
+def _serve_errors(done, fp64_max_n=2048, mixed_max_n=4096):
+    """Population error per request: FP64 against the exact propagator, mixed
+    against the same request run alone in FP64."""
+    import scipy.linalg as sl
+    from dynemol_amd import hip
+    from dynemol_amd.serve import Request, Server
+    err64, errmx, unchecked = [], [], 0
+    with hip.Context(0) as ctx:
+        for r in done:
+            if r.mixed and r.basis.N <= mixed_max_n:
+                again = Request(rid=r.rid, basis=r.basis, coords=r.coords, psi_bra=None, psi_ket=r.psi_ket,
+                                ao_init=True, steps=r.steps, n_frag=r.n_frag, mixed=False)
+                ref = Server(0, hbm_budget_bytes=96 << 30).run([again])[0]
+                errmx.append(max(float(np.abs(r.pops[q] - ref.pops[q]).max()) for q in range(len(r.psi_ket))))
+            elif not r.mixed and r.basis.N <= fp64_max_n:
+                s = hip.System(ctx, r.basis, n_wp=1, n_frag=r.n_frag)
+                s.build(r.coords)
+                S, h = s.matrix(hip.DH_MAT_S), s.matrix(hip.DH_MAT_H)
+                s.close()
+                E, C = sl.eigh(h, S)
+                ph = np.exp(-1j * E * r.steps * r.dt / hip.H_BAR)
+                e = 0.0
+                for q, c0 in enumerate(r.psi_ket):
+                    ke = C @ (ph * (C.T @ (S @ c0)))
+                    z = np.conj(ke) * (S @ ke)
+                    pe = np.array([z[r.basis.fragment == f].real.sum() for f in range(r.n_frag)] + [z.real.sum()])
+                    e = max(e, float(np.abs(r.pops[q] - pe).max()))
+                err64.append(e)
+            else:
+                unchecked += 1
+    log(f"serve errors: fp64 vs exact max {max(err64, default=0):.2e} over {len(err64)}, "
+        f"mixed vs fp64 max {max(errmx, default=0):.2e} over {len(errmx)}, {unchecked} unchecked")
+    return {"pop_err_max": max(err64 + errmx, default=None), "pop_err_max_fp64_vs_exact": max(err64, default=None),
+            "pop_err_max_mixed_vs_fp64": max(errmx, default=None), "checked_fp64": len(err64),
+            "checked_mixed": len(errmx), "unchecked": unchecked}
 
 
 def _exact(S, h, psi0, t):
@@ -277,9 +322,10 @@ def _exact(S, h, psi0, t):
 
 def run_many_systems(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
     """Many systems in one batch: C/H/N/O clusters, N log-uniform in [256, 2048],
-    4 wavepackets each (random S-normalised superpositions of 8 frontier orbitals),
-    grouped launches per Chebyshev term; sampled systems checked against the exact
-    propagator."""
+    4 wavepackets each (random S-normalised superpositions of 8 frontier orbitals
+    from the device eigensolver), grouped launches per Chebyshev term; sampled
+    systems checked against the exact propagator of their device-built S and h
+    (untimed)."""
     import scipy.linalg as sl
     from dynemol_amd import hip
     from dynemol_amd.structures import heavy_cluster, log_uniform_int
@@ -294,53 +340,55 @@ def run_many_systems(n_sys=256, steps=500, seed=3, n_wp=4, check=6):
             b = st.basis()
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
             s.build(st.coords)
-            S, h = s.matrix(hip.DH_MAT_S), s.matrix(hip.DH_MAT_H)
             mid = N // 2
-            E, C = sl.eigh(h, S, subset_by_index=[mid - 4, mid + 3])
+            E, C = s.eigenpairs(mid - 3, mid + 4)
             cs = []
             for w in range(n_wp):
-                c = C @ (rng.normal(size=8) + 1j * rng.normal(size=8))
-                c /= np.sqrt(np.real(np.conj(c) @ (S @ c)))
+                a = rng.normal(size=8) + 1j * rng.normal(size=8)
+                c = C @ (a / np.linalg.norm(a))
                 s.set_initial_state(w, c, c)
                 cs.append(c)
             systems.append(s)
             init.append(cs)
-            if (i + 1) % 32 == 0:
+            if (i + 1) % 64 == 0:
                 log(f"many systems: {i + 1} systems built")
-            mats.append((S, h) if i < check else None)
+            mats.append((s.matrix(hip.DH_MAT_S), s.matrix(hip.DH_MAT_H)) if i < check else None)
         out["setup_s"] = round(time.perf_counter() - t0, 2)
         out["n_min"] = min(s.N for s in systems)
         out["n_max"] = max(s.N for s in systems)
         out["n_mean"] = round(float(np.mean([s.N for s in systems])), 1)
         batch = hip.Batch(ctx, systems)
-        t = 0.0
-        batch.propagate(t, t + DT_PS)  # first step: includes the grouped task tables
-        t += DT_PS
+        batch.propagate(0.0, DT_PS)  # first step: task tables, kernel choice, graph capture
         t0 = time.perf_counter()
         for step in range(steps - 1):
-            batch.propagate(t, t + DT_PS)
-            t += DT_PS
+            batch.propagate(0.0, DT_PS)
             if (step + 2) % 100 == 0:
                 log(f"many systems: step {step + 2}")
         el = time.perf_counter() - t0
         out["wavepacket_steps_per_s"] = round(n_sys * n_wp * (steps - 1) / el, 1)
         out["ms_per_step"] = round(1e3 * el / (steps - 1), 3)
+        out["batch_kernel"] = batch.tuning()
         errs, perr = [], []
         for i in range(check):
             S, h = mats[i]
+            E, C = sl.eigh(h, S)
             for w in range(n_wp):
-                ke = _exact(S, h, init[i][w], t)
+                ke = C @ (np.exp(-1j * E * steps * DT_PS / hip.H_BAR) * (C.T @ (S @ init[i][w])))
                 _, k = systems[i].wavepacket(w)
                 errs.append(float(np.abs(k - ke).max() / np.abs(ke).max()))
                 _, pops, _ = systems[i].observables(w)
-                tot = float(np.real(np.conj(ke) @ (S @ ke)))
-                perr.append(abs(float(pops[-1]) - tot))
+                z = np.conj(ke) * (S @ ke)
+                frag = systems[i].basis.fragment
+                pe = np.array([z[frag == f].real.sum() for f in range(2)] + [z.real.sum()])
+                perr.append(float(np.abs(pops - pe).max()))
         out["checked_systems"] = check
         out["max_rel_psi_err_vs_exact"] = max(errs)
-        out["max_total_pop_err_vs_exact"] = max(perr)
+        out["max_pop_err_vs_exact"] = max(perr)
         batch.close()
         for s in systems:
             s.close()
+    log(f"many systems: {out['wavepacket_steps_per_s']} wavepacket-steps/s, {out['ms_per_step']} ms per step, "
+        f"max pop err {out['max_pop_err_vs_exact']:.1e}")
     return out
 
 
@@ -496,8 +544,10 @@ def main():
                     help="skip the diabatic-Ehrenfest kernel timing after the drop-in steps")
     ap.add_argument("--batched-wp", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=50)
-    ap.add_argument("--serve-requests", type=int, default=10)
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--many-steps", type=int, default=30,
+                    help="steps of the 256-system batch leg of the default run (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-sweep", type=str, default="",
                     help="comma-separated arrival rates: run only the serving leg at each and print one JSON line per rate")
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 drop-in leg")
@@ -527,7 +577,7 @@ def main():
     if args.serve_sweep:
         for lam in (float(x) for x in args.serve_sweep.split(",")):
             log(f"serve sweep: rate {lam}/s")
-            print(json.dumps(run_serve(args.serve_requests, lam)), flush=True)
+            print(json.dumps(run_serve(args.serve_requests, lam, check=False)), flush=True)
         return
     if args.calibrate_copy > 0:
         from dynemol_amd import hip
@@ -578,7 +628,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded TiO2 slab + C16H20 geometry, EHT parameters of the reference table)",
         "config": {"workload": "FP64 drop-in: TiO2 anatase slab + C16H20, N=4096, 2 wavepackets (electron LUMO, "
-                               "hole HOMO), 0.5 fs steps, S/h/Cholesky/Lanczos rebuilt every 10 steps (jitter 0.02 A)",
+                               "hole HOMO), 0.5 fs steps, S/h/Cholesky/Lanczos rebuilt every 10 steps (jitter 0.02 A), "
+                               "the step loop on the device (dh_propagate_steps)",
                    "n_orbitals": N, "wavepackets": 2, "rebuild_every": REBUILD_EVERY},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -598,6 +649,7 @@ def main():
         "ehrenfest": g["ehrenfest"],
         "eigenpairs": g["eigenpairs"],
         "batched": None,
+        "many_systems": None,
         "serve": None,
     }
     if not args.no_extra:
@@ -605,6 +657,9 @@ def main():
         out["batched"] = run_batched(st, b, S, args.batched_wp, args.batched_steps)
         out["batched"]["fp64_vs_dropin"] = round(out["batched"]["fp64_value"] / value, 3)
         out["batched"]["mixed_vs_dropin"] = round(out["batched"]["mixed_value"] / value, 3)
+        if args.many_steps > 0:
+            log("many-systems leg")
+            out["many_systems"] = run_many_systems(steps=args.many_steps)
         if args.serve_requests > 0:
             log("serve leg")
             out["serve"] = run_serve(args.serve_requests, args.serve_rate)

@@ -2,6 +2,7 @@
 // propagationelhl_gpucaller_ replacement.  Host orchestration only; every
 // arithmetic step runs in the kernels of overlap.hip, dense.hip and cheb.hip.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -130,6 +131,7 @@ struct dh_system {
     float* x32[2] = {nullptr, nullptr};
     bool h32_ready = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evpool;  // per-step series events of dh_propagate_steps (timing on)
 };
 
 #define CHECK_ARG(cond, msg)                  \
@@ -215,6 +217,7 @@ extern "C" int dh_system_destroy(dh_system* s) {
         if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out, s->stage,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
@@ -257,6 +260,14 @@ extern "C" long long dh_system_bytes(int n_orb, int n_wp) {
     b += (4 * R + 3LL * n_wp + n_wp) * ld * 16;             // phi, x0, x1, acc, out, hk
     b += (long long)partial_cplx_for(n_orb, n_wp) * 16;    // partials
     b += 3LL * ld * 16 + (long long)KMAX_ALLOC * 24;      // Lanczos, Bessel
+    return b;
+}
+
+extern "C" long long dh_system_bytes_prec(int n_orb, int n_wp, int precision) {
+    const long long ld = round_ld(n_orb);
+    long long b = dh_system_bytes(n_orb, n_wp);
+    if (precision != 0) b += ld * ld * 4;            // H32
+    if (precision == 1) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
     return b;
 }
 
@@ -989,30 +1000,12 @@ static void launch_outputs(dh_system* s) {
     launch_observables(s->N, W, s->n_frag, s->frag, s->out, s->phi, s->hk, s->pops, s->erg, s->norms, st);
 }
 
-extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_terms) {
-    CHECK_ARG(s, "dh_propagate: null system");
-    if (!s->ops_ready) {
-        dh::set_error("dh_propagate: operators not built");
-        return DH_ERR_STATE;
-    }
-    if (!s->state_set) {
-        dh::set_error("dh_propagate: wavepackets not set");
-        return DH_ERR_STATE;
-    }
-    DH_HIP(hipSetDevice(s->ctx->device));
+// One step of dt (eV^-1) with the norm check: the term series, the new state's
+// norms and the outputs, then one host wait; a step whose norm drifts is
+// redone with wider bounds (at most 4 attempts).  phi holds the current state.
+static int propagate_checked(dh_system* s, double dt, int* n_terms) {
     hipStream_t st = s->ctx->stream;
-    const double dt = (t_max - t_init) / H_BAR;
     const int R = 2 * s->W;
-    if (s->phi_dirty) {
-        convert_ao_to_phi(s);
-        s->phi_dirty = false;
-    }
-    if (!(dt > 0.0)) {
-        launch_outputs(s);
-        DH_HIP(hipStreamSynchronize(st));
-        if (n_terms) *n_terms = 0;
-        return DH_OK;
-    }
     launch_column_norms(s->N, R, s->phi, s->norms0, st);
     std::vector<double> n0(R), n1(R);
     // FP64 norm drift of one step that signals bounds not enclosing the
@@ -1065,6 +1058,186 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
     dh::set_error("dh_propagate: Chebyshev series lost the norm (drift %g) after widening the spectral bounds",
                   s->stats[4]);
     return DH_ERR_DIVERGED;
+}
+
+static int check_ready(dh_system* s, const char* who) {
+    if (!s->ops_ready) {
+        dh::set_error("%s: operators not built", who);
+        return DH_ERR_STATE;
+    }
+    if (!s->state_set) {
+        dh::set_error("%s: wavepackets not set", who);
+        return DH_ERR_STATE;
+    }
+    return DH_OK;
+}
+
+extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_terms) {
+    CHECK_ARG(s, "dh_propagate: null system");
+    int rc = check_ready(s, "dh_propagate");
+    if (rc) return rc;
+    DH_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const double dt = (t_max - t_init) / H_BAR;
+    if (s->phi_dirty) {
+        convert_ao_to_phi(s);
+        s->phi_dirty = false;
+    }
+    if (!(dt > 0.0)) {
+        launch_outputs(s);
+        DH_HIP(hipStreamSynchronize(st));
+        if (n_terms) *n_terms = 0;
+        return DH_OK;
+    }
+    return propagate_checked(s, dt, n_terms);
+}
+
+// Operators from the coordinates already in s->coords (device): S and h, then
+// the orthogonalisation (dh_build_operators after dh_set_coords).
+// The stream is idle when this starts (the interval before it ended with a host
+// wait), so the host time to the bounds, which wait for the device, is the
+// rebuild's device time: stats[8] accumulates it.
+static int rebuild_from_device_coords(dh_system* s, double cutoff, double s_drop) {
+    const auto t0 = std::chrono::steady_clock::now();
+    s->stats[7] += 1;
+    int rc = ensure_ws(s->ctx, s->ld);
+    if (rc) return rc;
+    s->ctx->ws_owner = s;
+    if (launch_build_SH(s->bd, s->N, s->ld, s->ctx->S, s->ctx->h, cutoff, s_drop, s->ctx->stream)) {
+        dh::set_error("dh_propagate_steps: kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        return DH_ERR_HIP;
+    }
+    rc = dh_orthogonalize(s, 0);
+    s->stats[8] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// Device buffers of one dh_propagate_steps call, freed on every return path.
+struct StepsScratch {
+    double *geo = nullptr, *n0 = nullptr, *n1 = nullptr, *pops = nullptr, *erg = nullptr;
+    cplx *ck_phi = nullptr, *ck_out = nullptr;
+    ~StepsScratch() {
+        for (void* p : {(void*)geo, (void*)n0, (void*)n1, (void*)pops, (void*)erg, (void*)ck_phi, (void*)ck_out})
+            if (p) hipFree(p);
+    }
+};
+
+extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int rebuild_every, const double* coords,
+                                  double cutoff, double s_drop, double* pops, double* erg, int* n_terms) {
+    CHECK_ARG(s, "dh_propagate_steps: null system");
+    CHECK_ARG(n_steps > 0 && dt_ps > 0.0 && rebuild_every >= 0, "dh_propagate_steps: need n_steps > 0, dt > 0, "
+              "rebuild_every >= 0");
+    const int n_geo = rebuild_every > 0 ? (n_steps - 1) / rebuild_every : 0;
+    CHECK_ARG(n_geo == 0 || (coords && s->n_atoms > 0), "dh_propagate_steps: rebuilds need coordinates of a system "
+              "built from atoms");
+    int rc = check_ready(s, "dh_propagate_steps");
+    if (rc) return rc;
+    DH_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const int W = s->W, R = 2 * W, NP = s->n_frag + 1;
+    const double dt = dt_ps / H_BAR;
+    StepsScratch d;
+    DALLOC(d.n0, (size_t)n_steps * R);
+    DALLOC(d.n1, (size_t)n_steps * R);
+    DALLOC(d.pops, (size_t)n_steps * W * NP);
+    DALLOC(d.erg, (size_t)n_steps * W * 2);
+    DALLOC(d.ck_phi, (size_t)s->ld * R);
+    DALLOC(d.ck_out, (size_t)s->ld * 3 * W);
+    const size_t geo_n = (size_t)s->n_atoms * 3;
+    if (n_geo > 0) {
+        DALLOC(d.geo, geo_n * n_geo);
+        // every geometry goes to the device once; a rebuild reads its slice there
+        DH_HIP(hipMemcpyAsync(d.geo, coords, sizeof(double) * geo_n * n_geo, hipMemcpyHostToDevice, st));
+    }
+    std::vector<double> h0((size_t)n_steps * R), h1((size_t)n_steps * R);
+    const double drift_tol = s->precision == 1 ? 1e-4 : 1e-9;
+    int last_terms = 0;
+    for (int j = 0; j < n_steps;) {
+        if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) {
+            DH_HIP(hipMemcpyAsync(s->coords, d.geo + geo_n * (j / rebuild_every - 1), sizeof(double) * geo_n,
+                                  hipMemcpyDeviceToDevice, st));
+            s->have_coords = true;
+            if ((rc = rebuild_from_device_coords(s, cutoff, s_drop))) return rc;
+        }
+        const int jend = rebuild_every > 0 ? std::min(n_steps, (j / rebuild_every + 1) * rebuild_every) : n_steps;
+        if (s->phi_dirty) {
+            convert_ao_to_phi(s);
+            s->phi_dirty = false;
+        }
+        // state at the start of the interval, for a redo of its steps one by one
+        DH_HIP(hipMemcpyAsync(d.ck_phi, s->phi, sizeof(cplx) * s->ld * R, hipMemcpyDeviceToDevice, st));
+        DH_HIP(hipMemcpyAsync(d.ck_out, s->out, sizeof(cplx) * s->ld * 3 * W, hipMemcpyDeviceToDevice, st));
+        if ((rc = ensure_coefficients(s, dt))) return rc;
+        if (s->timing)
+            while (s->evpool.size() < 2 * (size_t)(jend - j)) {
+                hipEvent_t e = nullptr;
+                DH_HIP(hipEventCreate(&e));
+                s->evpool.push_back(e);
+            }
+        for (int i = j; i < jend; ++i) {
+            launch_column_norms(s->N, R, s->phi, d.n0 + (size_t)i * R, st);
+            if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * (i - j)], st));
+            if ((rc = chebyshev_series(s))) return rc;
+            if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * (i - j) + 1], st));
+            std::swap(s->phi, s->acc);
+            launch_column_norms(s->N, R, s->phi, d.n1 + (size_t)i * R, st);
+            if (s->precision == 1)
+                launch_renorm_columns(s->phi, s->N, R, d.n0 + (size_t)i * R, d.n1 + (size_t)i * R, st);
+            launch_outputs(s);
+            DH_HIP(hipMemcpyAsync(d.pops + (size_t)i * W * NP, s->pops, sizeof(double) * W * NP,
+                                  hipMemcpyDeviceToDevice, st));
+            DH_HIP(hipMemcpyAsync(d.erg + (size_t)i * W * 2, s->erg, sizeof(double) * W * 2, hipMemcpyDeviceToDevice,
+                                  st));
+        }
+        DH_HIP(hipGetLastError());
+        const size_t cnt = (size_t)(jend - j) * R;
+        DH_HIP(hipMemcpyAsync(h0.data() + (size_t)j * R, d.n0 + (size_t)j * R, sizeof(double) * cnt,
+                              hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(h1.data() + (size_t)j * R, d.n1 + (size_t)j * R, sizeof(double) * cnt,
+                              hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        float series_ms = 0.0f;
+        if (s->timing)
+            for (int i = 0; i < jend - j; ++i) {
+                float ms = 0.0f;
+                DH_HIP(hipEventElapsedTime(&ms, s->evpool[2 * i], s->evpool[2 * i + 1]));
+                series_ms += ms;
+            }
+        double drift = 0.0;
+        for (size_t e = (size_t)j * R; e < (size_t)jend * R; ++e) {
+            if (h0[e] > 0) drift = std::max(drift, std::fabs(h1[e] - h0[e]) / h0[e]);
+            if (!std::isfinite(h1[e])) drift = INFINITY;
+        }
+        if (drift < drift_tol) {
+            s->stats[4] = drift;
+            s->stats[0] = s->K + 1;
+            s->stats[1] += jend - j;
+            s->stats[2] += (double)(s->K + 3) * (jend - j);
+            if (s->timing) {
+                s->stats[5] += series_ms;
+                s->stats[6] += (double)s->K * (jend - j);
+            }
+            last_terms = s->K + 1;
+        } else {
+            // a step left the bounds: back to the interval's start, then step by
+            // step with the checked step (wider bounds where needed)
+            DH_HIP(hipMemcpyAsync(s->phi, d.ck_phi, sizeof(cplx) * s->ld * R, hipMemcpyDeviceToDevice, st));
+            DH_HIP(hipMemcpyAsync(s->out, d.ck_out, sizeof(cplx) * s->ld * 3 * W, hipMemcpyDeviceToDevice, st));
+            for (int i = j; i < jend; ++i) {
+                if ((rc = propagate_checked(s, dt, &last_terms))) return rc;
+                DH_HIP(hipMemcpyAsync(d.pops + (size_t)i * W * NP, s->pops, sizeof(double) * W * NP,
+                                      hipMemcpyDeviceToDevice, st));
+                DH_HIP(hipMemcpyAsync(d.erg + (size_t)i * W * 2, s->erg, sizeof(double) * W * 2,
+                                      hipMemcpyDeviceToDevice, st));
+            }
+        }
+        j = jend;
+    }
+    if (pops) DH_HIP(hipMemcpyAsync(pops, d.pops, sizeof(double) * n_steps * W * NP, hipMemcpyDeviceToHost, st));
+    if (erg) DH_HIP(hipMemcpyAsync(erg, d.erg, sizeof(double) * n_steps * W * 2, hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    if (n_terms) *n_terms = last_terms;
+    return DH_OK;
 }
 
 extern "C" int dh_get_observables(dh_system* s, int wp, double* ao_bra, double* pops, double* erg) {
@@ -1557,33 +1730,73 @@ struct dh_batch {
     std::vector<int> Ks;
     int Kmax = 0;
     double stats[4] = {0, 0, 0, 0};  // steps, term launches, last max drift, retries
-    // the term series as one captured graph, kept while the term counts stay
-    // the same (its kernels read the per-step descriptors from d_args)
+    // the term series as one captured graph, kept while its key (term counts,
+    // kernel family, and the by-value arguments of the members that run their
+    // own series) stays the same; the grouped kernels read the per-step
+    // descriptors from d_args
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
-    std::vector<int> gKs;
-    // R in {2, 4, 8}: the terms run the symmetric half-storage kernel over all
-    // systems (phase-1 tasks: segments; phase-2 tasks: block x chunk), each
-    // system with the segment length its own single launch would use
-    bool sym = false;
+    std::vector<double> gkey;
+    // grouped term kernel: symmetric half-storage (R <= 8; phase-1 tasks:
+    // segments, phase-2 tasks: block x chunk, each system with the segment
+    // length its own single launch would use) or dense column blocks, as the
+    // batch autotuner chose for this R
+    bool sym_ok = false, sym = false;
     int *d_t1_desc = nullptr, *d_t1_idx = nullptr, *d_t2_desc = nullptr, *d_t2_idx = nullptr, *d_sysS = nullptr;
     std::vector<int> prefix1, prefix2;
+    // members in a precision mode other than FP64 run their own term series
+    // (FP32 matrix-core GEMM or FP32-stored operator) inside the batch step
+    std::vector<char> solo;
+    // host copies of the last uploaded tables: uploaded again only on change
+    std::vector<MatvecArgs> h_args;
+    std::vector<NormTask> h_norms;
+    std::vector<CopyTask> h_copy;
+    std::vector<ObsTask> h_obs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-// Chebyshev terms of a batch: systems with K_s >= k are the first `active` of `order`
+// Batch term kernel per R, chosen once per process by timing both grouped
+// families on the first batch of that R (-1: not yet tuned, 0: dense column
+// blocks, 1: symmetric half storage); every later batch of that R runs the
+// same kernels, so a system's arithmetic does not depend on its batch.
+static int g_batch_family[6] = {-1, -1, -1, -1, -1, -1};
+static std::mutex g_batch_family_mu;
+static int batch_family_slot(int R) { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : R == 32 ? 5 : 0; }
+
+static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k, hipStream_t st) {
+    const int n = b->n;
+    if (sym)
+        launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
+                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, st);
+    else
+        launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], b->R, k,
+                              st);
+}
+
+// Chebyshev terms of a batch: grouped members with K_s >= k are the first
+// `active` of `order`; members in other precision modes run their own series
 static void batch_series_launches(dh_batch* b, hipStream_t st) {
-    const int n = b->n, R = b->R;
+    const int n = b->n;
     int active = n;
     for (int k = 1; k <= b->Kmax; ++k) {
         while (active > 0 && b->Ks[b->order[active - 1]] < k) --active;
         const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
-        if (b->sym)
-            launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
-                               b->d_t2_idx, b->prefix2[active], b->d_sysS, R, k, st);
-        else
-            launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], R, k,
-                                  st);
+        batch_term_launch(b, b->sym, kind, active, k, st);
     }
+    for (int i = 0; i < n; ++i)
+        if (b->solo[i]) series_launches(b->sys[i]);
+}
+
+static std::vector<double> batch_graph_key(const dh_batch* b) {
+    std::vector<double> key(b->Ks.begin(), b->Ks.end());
+    key.push_back(b->sym ? 1.0 : 0.0);
+    for (int i = 0; i < b->n; ++i)
+        if (b->solo[i]) {
+            const dh_system* s = b->sys[i];
+            key.insert(key.end(), {(double)i, s->coef_c, s->coef_r, (double)s->precision,
+                                   (double)reinterpret_cast<uintptr_t>(s->phi)});
+        }
+    return key;
 }
 
 static int batch_series(dh_batch* b, hipStream_t st) {
@@ -1591,7 +1804,8 @@ static int batch_series(dh_batch* b, hipStream_t st) {
         batch_series_launches(b, st);
         return DH_OK;
     }
-    if (!b->gexec || b->gKs != b->Ks) {
+    std::vector<double> key = batch_graph_key(b);
+    if (!b->gexec || b->gkey != key) {
         if (b->gexec) {
             DH_HIP(hipGraphExecDestroy(b->gexec));
             b->gexec = nullptr;
@@ -1612,7 +1826,7 @@ static int batch_series(dh_batch* b, hipStream_t st) {
             dh::set_error("batch term series graph instantiation failed: %s", hipGetErrorString(ei));
             return DH_ERR_HIP;
         }
-        b->gKs = b->Ks;
+        b->gkey = key;
     }
     DH_HIP(hipGraphLaunch(b->gexec, st));
     return DH_OK;
@@ -1629,6 +1843,8 @@ extern "C" int dh_batch_destroy(dh_batch* b) {
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
     void* ptrs[] = {b->d_args,  b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs,
                     b->d_nval,  b->d_t1_desc,   b->d_t1_idx,   b->d_t2_desc, b->d_t2_idx, b->d_sysS};
     for (void* p : ptrs)
@@ -1654,6 +1870,7 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
     b->W = W;
     b->R = R;
     b->n = n;
+    b->solo.assign(n, 0);
     long tasks = 0;
     const int cpb0 = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
     for (auto* s : b->sys) tasks += (s->N + cpb0 - 1) / cpb0;
@@ -1665,15 +1882,16 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
             return rc;                                  \
         }                                               \
     } while (0)
+    // term tasks (grouped members only) and output tasks (every member)
     BALLOC(b->d_args, (size_t)dh_batch::NDESC * n);
-    BALLOC(b->d_task_desc, (size_t)tasks);
-    BALLOC(b->d_task_blk, (size_t)tasks);
+    BALLOC(b->d_task_desc, (size_t)2 * tasks);
+    BALLOC(b->d_task_blk, (size_t)2 * tasks);
     BALLOC(b->d_norms, (size_t)2 * n * R);
     BALLOC(b->d_copy, (size_t)n);
     BALLOC(b->d_obs, (size_t)n * W);
     BALLOC(b->d_nval, (size_t)2 * n * R);
-    b->sym = sym_supported(R, TRI_FULL);
-    if (b->sym) {
+    b->sym_ok = sym_supported(R, TRI_FULL);
+    if (b->sym_ok) {
         long t1 = 0, t2 = 0;
         std::vector<int> sS(n);
         for (int i = 0; i < n; ++i) {
@@ -1692,6 +1910,11 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
         }
     }
 #undef BALLOC
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        dh_batch_destroy(b);
+        dh::set_error("dh_batch_create: event creation failed");
+        return DH_ERR_HIP;
+    }
     *out = b;
     return DH_OK;
 }
@@ -1742,6 +1965,58 @@ static MatvecArgs batch_desc(dh_system* s, int kind) {
     return a;
 }
 
+// host -> device table copy, only when the content changed since the last copy
+// (from the batch's own copy, which outlives the asynchronous transfer)
+template <class T>
+static int upload_if_changed(std::vector<T>& cache, const std::vector<T>& v, T* dst, hipStream_t st) {
+    if (cache.size() == v.size() && (v.empty() || std::memcmp(cache.data(), v.data(), sizeof(T) * v.size()) == 0))
+        return DH_OK;
+    DH_HIP(hipStreamSynchronize(st));  // the previous upload from the cache is done
+    cache = v;
+    if (!cache.empty()) DH_HIP(hipMemcpyAsync(dst, cache.data(), sizeof(T) * cache.size(), hipMemcpyHostToDevice, st));
+    return DH_OK;
+}
+
+// Times the first grouped term (k = 1: reads phi, writes x1 and acc, which the
+// series then overwrites) with both kernel families and keeps the faster for R.
+static int batch_autotune(dh_batch* b, hipStream_t st) {
+    const int slot = batch_family_slot(b->R);
+    if (b->prefix[b->n] == 0) {  // every member runs its own series: nothing to time
+        b->sym = false;
+        return DH_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_batch_family_mu);
+        if (g_batch_family[slot] >= 0) {
+            b->sym = g_batch_family[slot] == 1 && b->sym_ok;
+            return DH_OK;
+        }
+    }
+    int pick = 0;
+    if (b->sym_ok) {
+        float ms[2] = {0.f, 0.f};
+        for (int fam = 0; fam < 2; ++fam) {
+            for (int w = 0; w < 2; ++w) batch_term_launch(b, fam == 1, dh_batch::D_K1, b->n, 1, st);
+            DH_HIP(hipEventRecord(b->ev0, st));
+            for (int r = 0; r < 5; ++r) batch_term_launch(b, fam == 1, dh_batch::D_K1, b->n, 1, st);
+            DH_HIP(hipEventRecord(b->ev1, st));
+            DH_HIP(hipEventSynchronize(b->ev1));
+            DH_HIP(hipEventElapsedTime(&ms[fam], b->ev0, b->ev1));
+        }
+        pick = ms[1] < ms[0] ? 1 : 0;
+    }
+    std::lock_guard<std::mutex> lk(g_batch_family_mu);
+    if (g_batch_family[slot] < 0) g_batch_family[slot] = pick;
+    b->sym = g_batch_family[slot] == 1 && b->sym_ok;
+    return DH_OK;
+}
+
+extern "C" int dh_batch_tuning(dh_batch* b, int* family) {
+    CHECK_ARG(b && family, "dh_batch_tuning: null argument");
+    *family = b->sym ? 1 : 0;
+    return DH_OK;
+}
+
 extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int* max_terms) {
     CHECK_ARG(b, "dh_batch_propagate: null batch");
     DH_HIP(hipSetDevice(b->ctx->device));
@@ -1753,13 +2028,12 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
             dh::set_error("dh_batch_propagate: every system needs operators and wavepackets");
             return DH_ERR_STATE;
         }
-        if (s->precision != 0) {
-            dh::set_error("dh_batch_propagate: batches run the FP64 term kernel");
-            return DH_ERR_STATE;
-        }
     }
     if (!(dt > 0.0)) return DH_OK;
-    for (auto* s : b->sys) {
+    std::vector<char> solo(n);
+    for (int i = 0; i < n; ++i) {
+        dh_system* s = b->sys[i];
+        solo[i] = s->precision != 0;
         if (s->phi_dirty) {
             convert_ao_to_phi(s);
             s->phi_dirty = false;
@@ -1767,11 +2041,14 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
         int rc = ensure_coefficients(s, dt);
         if (rc) return rc;
     }
-    // task table by term count (descending)
+    // task tables by term count (descending); members in other precision modes
+    // add no grouped term tasks
     std::vector<int> Ks(n);
     for (int i = 0; i < n; ++i) Ks[i] = b->sys[i]->K;
-    if (Ks != b->Ks) {
+    const bool retable = Ks != b->Ks || solo != b->solo || b->prefix.empty();
+    if (retable) {
         b->Ks = Ks;
+        b->solo = solo;
         b->order.resize(n);
         for (int i = 0; i < n; ++i) b->order[i] = i;
         std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return Ks[x] > Ks[y]; });
@@ -1779,42 +2056,59 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
         b->prefix.assign(1, 0);
         const int cpb = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
         for (int q : b->order) {
-            const int nb = (b->sys[q]->N + cpb - 1) / cpb;
-            for (int blk = 0; blk < nb; ++blk) {
+            const int nbq = solo[q] ? 0 : (b->sys[q]->N + cpb - 1) / cpb;
+            for (int blk = 0; blk < nbq; ++blk) {
                 td.push_back(q);
                 tb.push_back(blk);
             }
             b->prefix.push_back((int)td.size());
         }
+        // output tasks: every member, after the term tasks
+        const int nterm = (int)td.size();
+        for (int q = 0; q < n; ++q)
+            for (int blk = 0; blk < (b->sys[q]->N + cpb - 1) / cpb; ++blk) {
+                td.push_back(q);
+                tb.push_back(blk);
+            }
+        b->prefix.push_back(nterm);  // prefix[n + 1]: start of the output tasks
+        b->prefix.push_back((int)td.size());
         DH_HIP(hipMemcpyAsync(b->d_task_desc, td.data(), sizeof(int) * td.size(), hipMemcpyHostToDevice, st));
         DH_HIP(hipMemcpyAsync(b->d_task_blk, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
-        if (b->sym) {  // the same system order for the symmetric term's two phases
+        if (b->sym_ok) {  // the same member order for the symmetric term's two phases
             std::vector<int> d1, i1, d2, i2;
             b->prefix1.assign(1, 0);
             b->prefix2.assign(1, 0);
             for (int q : b->order) {
-                int S = 0;
-                const int nseg = sym_term_segments(b->sys[q]->ld, &S);
-                for (int g = 0; g < nseg; ++g) {
-                    d1.push_back(q);
-                    i1.push_back(g);
-                }
-                const int nbq = b->sys[q]->ld / TILE;
-                for (int v = 0; v < nbq * R; ++v) {
-                    d2.push_back(q);
-                    i2.push_back(v);
+                if (!solo[q]) {
+                    int S = 0;
+                    const int nseg = sym_term_segments(b->sys[q]->ld, &S);
+                    for (int g = 0; g < nseg; ++g) {
+                        d1.push_back(q);
+                        i1.push_back(g);
+                    }
+                    const int nbq = b->sys[q]->ld / TILE;
+                    for (int v = 0; v < nbq * R; ++v) {
+                        d2.push_back(q);
+                        i2.push_back(v);
+                    }
                 }
                 b->prefix1.push_back((int)d1.size());
                 b->prefix2.push_back((int)d2.size());
             }
-            DH_HIP(hipMemcpyAsync(b->d_t1_desc, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice, st));
-            DH_HIP(hipMemcpyAsync(b->d_t1_idx, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice, st));
-            DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
-            DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
+            if (!d1.empty()) {
+                DH_HIP(hipMemcpyAsync(b->d_t1_desc, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice, st));
+                DH_HIP(hipMemcpyAsync(b->d_t1_idx, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice, st));
+                DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
+                DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
+            }
         }
-        b->Kmax = Ks[b->order[0]];
+        b->Kmax = 0;
+        for (int i = 0; i < n; ++i)
+            if (!solo[i]) b->Kmax = std::max(b->Kmax, Ks[i]);
+        DH_HIP(hipStreamSynchronize(st));  // the task vectors above are pageable and local
     }
-    // per-system descriptors (pointers are re-read every step)
+    // per-member descriptors, norm tasks: uploaded when they change (bounds,
+    // buffers); pointers are read on the device every step
     std::vector<MatvecArgs> args((size_t)dh_batch::NDESC * n);
     for (int kind = 0; kind < dh_batch::NDESC; ++kind)
         for (int i = 0; i < n; ++i) args[(size_t)kind * n + i] = batch_desc(b->sys[i], kind);
@@ -1825,8 +2119,10 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
             nt[(size_t)n * R + (size_t)i * R + c] =
                 NormTask{b->sys[i]->acc, b->sys[i]->N, R, c, b->d_nval + (size_t)n * R + (size_t)i * R + c};
         }
-    DH_HIP(hipMemcpyAsync(b->d_args, args.data(), sizeof(MatvecArgs) * args.size(), hipMemcpyHostToDevice, st));
-    DH_HIP(hipMemcpyAsync(b->d_norms, nt.data(), sizeof(NormTask) * nt.size(), hipMemcpyHostToDevice, st));
+    if (int rc = upload_if_changed(b->h_args, args, b->d_args, st)) return rc;
+    if (int rc = upload_if_changed(b->h_norms, nt, b->d_norms, st)) return rc;
+    if (retable || b->gkey.empty())
+        if (int rc = batch_autotune(b, st)) return rc;
     launch_grouped_norms(b->d_norms, n * R, st);
     if (int rc = batch_series(b, st)) return rc;
     launch_grouped_norms(b->d_norms + (size_t)n * R, n * R, st);
@@ -1835,31 +2131,41 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     DH_HIP(hipStreamSynchronize(st));
     std::vector<int> ok(n, 1);
     double maxdrift = 0.0;
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
+        const double tol = b->sys[i]->precision == 1 ? 1e-4 : 1e-9;
         for (int c = 0; c < R; ++c) {
             const double n0 = nv[(size_t)i * R + c], n1 = nv[(size_t)n * R + (size_t)i * R + c];
             const double d = n0 > 0 ? std::fabs(n1 - n0) / n0 : 0.0;
-            if (!(d < 1e-9)) ok[i] = 0;
+            if (!(d < tol)) ok[i] = 0;
             if (std::isfinite(d)) maxdrift = std::max(maxdrift, d);
         }
+    }
     // commit acc -> phi for systems whose norm held
     std::vector<CopyTask> ct;
     for (int i = 0; i < n; ++i)
         if (ok[i]) ct.push_back(CopyTask{b->sys[i]->acc, b->sys[i]->phi, (long)b->sys[i]->N * R});
-    DH_HIP(hipMemcpyAsync(b->d_copy, ct.data(), sizeof(CopyTask) * ct.size(), hipMemcpyHostToDevice, st));
+    if (int rc = upload_if_changed(b->h_copy, ct, b->d_copy, st)) return rc;
     launch_grouped_copy(b->d_copy, (int)ct.size(), st);
-    // outputs of every system
-    const int all = b->prefix[n];
-    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_ENERGY * n, b->d_task_desc, b->d_task_blk, all, W, 0, st);
-    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LBRA * n, b->d_task_desc, b->d_task_blk, all, W, 0, st);
-    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LINVT * n, b->d_task_desc, b->d_task_blk, all, R, 0, st);
+    // the mixed mode's per-step FP64 norm restoration, as in its single step
+    for (int i = 0; i < n; ++i)
+        if (ok[i] && b->sys[i]->precision == 1)
+            launch_renorm_columns(b->sys[i]->phi, b->sys[i]->N, R, b->d_nval + (size_t)i * R,
+                                  b->d_nval + (size_t)n * R + (size_t)i * R, st);
+    // outputs of every member
+    const int o0 = b->prefix[n + 1], o1 = b->prefix[n + 2];
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_ENERGY * n, b->d_task_desc + o0, b->d_task_blk + o0, o1 - o0,
+                          W, 0, st);
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LBRA * n, b->d_task_desc + o0, b->d_task_blk + o0, o1 - o0, W,
+                          0, st);
+    launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LINVT * n, b->d_task_desc + o0, b->d_task_blk + o0, o1 - o0,
+                          R, 0, st);
     std::vector<ObsTask> ot((size_t)n * W);
     for (int i = 0; i < n; ++i) {
         dh_system* s = b->sys[i];
         for (int w = 0; w < W; ++w)
             ot[(size_t)i * W + w] = ObsTask{s->N, W, s->n_frag, w, s->frag, s->out, s->phi, s->hk, s->pops, s->erg, s->norms};
     }
-    DH_HIP(hipMemcpyAsync(b->d_obs, ot.data(), sizeof(ObsTask) * ot.size(), hipMemcpyHostToDevice, st));
+    if (int rc = upload_if_changed(b->h_obs, ot, b->d_obs, st)) return rc;
     launch_grouped_observables(b->d_obs, n * W, st);
     DH_HIP(hipGetLastError());
     DH_HIP(hipStreamSynchronize(st));
@@ -1881,7 +2187,7 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     b->stats[0] += 1;
     b->stats[1] += b->Kmax;
     b->stats[2] = maxdrift;
-    if (max_terms) *max_terms = b->Kmax + 1;
+    if (max_terms) *max_terms = *std::max_element(Ks.begin(), Ks.end()) + 1;
     return DH_OK;
 }
 

@@ -88,6 +88,19 @@ module dynemol_hip
             real(c_double), value :: t_init, t_max
             integer(c_int) :: n_terms
         end function
+        ! several nuclear steps and rebuilds in one call (Chebyshev_driver.f:102-169):
+        ! coords(3, n_atoms, (n_steps - 1) / rebuild_every), pops(n_frag + 1, n_wp, n_steps),
+        ! erg(n_wp, n_steps)
+        integer(c_int) function dh_propagate_steps(sys, n_steps, dt, rebuild_every, coords, cutoff, s_drop, &
+                                                   pops, erg, n_terms) bind(C, name="dh_propagate_steps")
+            import :: c_int, c_double, c_double_complex, c_ptr
+            type(c_ptr), value :: sys
+            integer(c_int), value :: n_steps, rebuild_every
+            real(c_double), value :: dt, cutoff, s_drop
+            real(c_double) :: coords(*), pops(*)
+            complex(c_double_complex) :: erg(*)
+            integer(c_int) :: n_terms
+        end function
         integer(c_int) function dh_get_observables(sys, wp, ao_bra, pops, erg) bind(C, name="dh_get_observables")
             import :: c_int, c_double, c_double_complex, c_ptr
             type(c_ptr), value :: sys

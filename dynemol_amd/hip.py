@@ -31,12 +31,13 @@ _vp = ctypes.c_void_p
 
 SYMBOLS = [
     "dh_last_error", "dh_version", "dh_init", "dh_finalize", "dh_system_create", "dh_system_destroy",
-    "dh_system_bytes", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
+    "dh_system_bytes", "dh_system_bytes_prec", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
     "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
-    "dh_chebyshev_work_bytes", "dh_chebyshev_apply", "dh_chebyshev_apply_noalloc",
+    "dh_chebyshev_work_bytes", "dh_chebyshev_apply", "dh_chebyshev_apply_noalloc", "dh_propagate_steps",
+    "dh_batch_tuning",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read", "dh_test_device_alloc", "dh_test_device_free",
@@ -76,6 +77,8 @@ def lib():
     L.dh_system_destroy.argtypes = [_vp]
     L.dh_system_bytes.restype = ctypes.c_longlong
     L.dh_system_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.dh_system_bytes_prec.restype = ctypes.c_longlong
+    L.dh_system_bytes_prec.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.dh_set_coords.argtypes = [_vp, _d]
     L.dh_build_operators.argtypes = [_vp, ctypes.c_double, ctypes.c_double]
     L.dh_load_operators.argtypes = [_vp, _d, _d, ctypes.c_int]
@@ -87,6 +90,8 @@ def lib():
     L.dh_get_wavepacket.argtypes = [_vp, ctypes.c_int, _d, _d]
     L.dh_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
     L.dh_get_observables.argtypes = [_vp, ctypes.c_int, _d, _d, _d]
+    L.dh_propagate_steps.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d, ctypes.c_double,
+                                     ctypes.c_double, _d, _d, _i]
     L.dh_ehrenfest_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _d, ctypes.c_int]
     L.dh_eigenpairs.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _d, _d, ctypes.c_int]
     L.dh_batch_set_graph.argtypes = [_vp, ctypes.c_int]
@@ -107,6 +112,7 @@ def lib():
     L.dh_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.dh_batch_destroy.argtypes = [_vp]
     L.dh_batch_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
+    L.dh_batch_tuning.argtypes = [_vp, _i]
     L.dh_test_force_matvec_variant.argtypes = [ctypes.c_int]
     L.dh_test_matvec_timing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
     L.dh_chebyshev_work_bytes.restype = ctypes.c_longlong
@@ -285,6 +291,23 @@ class System:
         _check(lib().dh_propagate(self.h, t_init, t_max, ctypes.byref(n)), "dh_propagate")
         return n.value
 
+    def propagate_steps(self, n_steps, dt, rebuild_every=0, geometries=None, cutoff=12.0, s_drop=0.0):
+        """n_steps steps of dt (ps) on the device, rebuilding the operators from
+        geometries[g] (n_atoms x 3) before step (g + 1) * rebuild_every.  Returns
+        (pops [n_steps, n_wp, n_frag + 1], erg [n_steps, n_wp] complex, terms)."""
+        n_geo = (n_steps - 1) // rebuild_every if rebuild_every > 0 else 0
+        geo = None
+        if n_geo > 0:
+            geo = np.ascontiguousarray(np.asarray(geometries, dtype=np.float64)[:n_geo])
+            assert geo.shape[0] == n_geo, f"need {n_geo} geometries"
+        pops = np.zeros((n_steps, self.n_wp, self.n_frag + 1))
+        erg = np.zeros((n_steps, self.n_wp, 2))
+        n = ctypes.c_int(0)
+        _check(lib().dh_propagate_steps(self.h, int(n_steps), float(dt), int(rebuild_every),
+                                        None if geo is None else geo.ctypes.data_as(_d), cutoff, s_drop, _p(pops),
+                                        _p(erg), ctypes.byref(n)), "dh_propagate_steps")
+        return pops, erg[..., 0] + 1j * erg[..., 1], n.value
+
     def observables(self, wp):
         ao = np.zeros(self.N, dtype=np.complex128)
         pops = np.zeros(self.n_frag + 1)
@@ -317,7 +340,7 @@ class System:
         _check(lib().dh_get_stats(self.h, _p(out)), "dh_get_stats")
         return {"terms": int(out[0]), "steps": int(out[1]), "matvecs": int(out[2]), "retries": int(out[3]),
                 "norm_drift": float(out[4]), "series_ms": float(out[5]), "series_launches": int(out[6]),
-                "rebuilds": int(out[7])}
+                "rebuilds": int(out[7]), "rebuild_ms_in_steps": float(out[8])}
 
     def tuning(self):
         out = np.zeros(7, dtype=np.int32)
@@ -381,6 +404,12 @@ class Batch:
         """Term series as one captured hipGraph per step (default) or as separate launches."""
         _check(lib().dh_batch_set_graph(self.h, 1 if on else 0), "dh_batch_set_graph")
 
+    def tuning(self):
+        """Grouped term kernel the batch autotuner chose: 'symmetric' or 'dense'."""
+        f = ctypes.c_int(0)
+        _check(lib().dh_batch_tuning(self.h, ctypes.byref(f)), "dh_batch_tuning")
+        return "symmetric" if f.value == 1 else "dense"
+
     def close(self):
         if self.h:
             lib().dh_batch_destroy(self.h)
@@ -409,8 +438,8 @@ def run_copy(nbytes, repeats=20):
     return ms.value
 
 
-def system_bytes(n_orb, n_wp):
-    return int(lib().dh_system_bytes(n_orb, n_wp))
+def system_bytes(n_orb, n_wp, precision=DH_PREC_FP64):
+    return int(lib().dh_system_bytes_prec(n_orb, n_wp, int(precision)))
 
 
 def chebyshev_work_bytes(N, R):

@@ -58,8 +58,10 @@ int dh_system_create(dh_context *ctx, int n_atoms, const int *first_orb, const i
                      const double *orb_zeta, const double *orb_coef, const double *orb_ip, const double *orb_kwh,
                      const double *orb_vshift, const int *orb_frag, int n_frag, int n_wp, dh_system **sys);
 int dh_system_destroy(dh_system *sys);
-/* Device bytes a system of this size holds (serving admission). */
+/* Device bytes a system of this size holds (serving admission), in FP64 and
+ * with the FP32 operator copies of a precision mode (dh_set_precision). */
 long long dh_system_bytes(int n_orb, int n_wp);
+long long dh_system_bytes_prec(int n_orb, int n_wp, int precision);
 
 /* Atomic coordinates, n_atoms x 3 row-major (x, y, z per atom), Angstrom
  * (system%coord, types_EHT.f).  */
@@ -124,6 +126,24 @@ int dh_set_eigenstate(dh_system *sys, int wp, int k, double *energy);
  * NULL) receives the Chebyshev term count used.
  */
 int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
+/*
+ * n_steps nuclear steps in one call, the loop of the driver (Chebyshev_driver.f:
+ * 102-169 calling ElHl_Chebyshev, ElHl_Chebyshev.f:180-283) with the state, the
+ * operators and the per-step outputs kept on the device: each step propagates
+ * every wavepacket by dt (ps) as dh_propagate does; with rebuild_every = r > 0
+ * the operators are rebuilt (S, h, Cholesky, H_orth, bounds) before steps r,
+ * 2r, ... from geometry g = step / r - 1 of coords (n_atoms x 3 per geometry,
+ * (n_steps - 1) / r geometries, all copied to the device at the start).  The
+ * host waits once per rebuild interval, where the norms of the interval's
+ * steps are checked; an interval with a drifting step is redone step by step
+ * with wider bounds.  pops (n_steps x n_wp x (n_frag + 1)) and erg (n_steps x
+ * n_wp x 2) receive each step's populations and energies (may be NULL); the
+ * final state is read as after dh_propagate.  Same kernels and results as
+ * n_steps calls of dh_propagate(sys, 0, dt) with dh_set_coords +
+ * dh_build_operators at the rebuild steps.
+ */
+int dh_propagate_steps(dh_system *sys, int n_steps, double dt, int rebuild_every, const double *coords,
+                       double cutoff, double s_drop, double *pops, double *erg, int *n_terms);
 /* Observables after dh_propagate for wavepacket wp:
  * ao_bra (N complex) = S^-1 Psi_t_bra, not conjugated (Taylor_gpu.cpp:716-718);
  * pops (n_frag + 1) = Populations: Re sum_frag conj(Psi_ket) Psi_bra, total
@@ -148,7 +168,8 @@ int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, dou
 int dh_ehrenfest_kernel(dh_system *sys, int wp_el, int wp_hl, const double *X, int ldx, double *K, int ldk);
 /* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
  * retries, last norm drift, term-series GPU ms (HIP events, when timing is
- * on), term launches timed, operator rebuilds, reserved]. */
+ * on), term launches timed, operator rebuilds, ms of the rebuilds inside
+ * dh_propagate_steps]. */
 int dh_get_stats(dh_system *sys, double *out9);
 /* Record HIP events around each step's Chebyshev term series (on the
  * context stream that runs them); no counterpart in the reference, whose
@@ -169,7 +190,8 @@ int dh_matvec_tuning(dh_system *sys, int *out7);
 /*
  * Multi-system batch: n systems of possibly different N with the same number
  * of wavepackets (1, 2, 4, 8 or 16), advanced together by one grouped launch per
- * Chebyshev term.  The systems stay owned by the caller; results are read per
+ * Chebyshev term (FP64 members; members in other precision modes run their own
+ * term kernels in the same step).  The systems stay owned by the caller; results are read per
  * system with dh_get_wavepacket / dh_get_observables.  No counterpart in the
  * reference (one system per process, ElHl_Chebyshev.f:148-291).
  */
@@ -177,6 +199,12 @@ typedef struct dh_batch dh_batch;
 int dh_batch_create(dh_context *ctx, int n, dh_system *const *systems, dh_batch **batch);
 int dh_batch_destroy(dh_batch *batch);
 int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_terms);
+/* Term kernel family the batch autotuner chose for this batch's wavepacket
+ * count (once per process, by timing both on the first batch of that count):
+ * 1 = symmetric half-storage tiles, 0 = dense column blocks.  Members in a
+ * precision mode other than FP64 run their own term kernels inside the batch
+ * step.  No counterpart in the reference. */
+int dh_batch_tuning(dh_batch *batch, int *family);
 /* Run each step's Chebyshev term launches as one captured hipGraph (on = 1,
  * the default; captured again when a system's term count changes) or as
  * individual launches (on = 0).  Results are identical either way.  No

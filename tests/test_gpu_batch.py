@@ -100,3 +100,95 @@ def test_batch_graph_replays_match_launches():
                     assert np.array_equal(sg.observables(w)[1], sl_.observables(w)[1])
         for on in (True, False):
             batches[on].close()
+
+
+def test_batch_32_systems_up_to_2048_vs_exact():
+    """32 C/H/N/O systems with N log-uniform in [256, 2048], 4 wavepackets each,
+    10 steps in one batch (the many-system configuration at a tenth of its
+    size); sampled systems against the exact propagator (FP64 bar)."""
+    from dynemol_amd.structures import log_uniform_int
+    rng = np.random.default_rng(33)
+    sizes = [log_uniform_int(rng, 256, 2048) for _ in range(32)]
+    sizes[0], sizes[1] = 2048, 256
+    _say(f"batch of 32 systems, N in [{min(sizes)}, {max(sizes)}]")
+    check = (0, 1, 7, 19)
+    with hip.Context(0) as ctx:
+        systems, kept = [], {}
+        for i, N in enumerate(sizes):
+            st = heavy_cluster(500 + i, N)
+            b = st.basis()
+            s = hip.System(ctx, b, n_wp=4, n_frag=2)
+            s.build(st.coords)
+            E, C = s.eigenpairs(N // 2 - 3, N // 2 + 4)
+            cs = []
+            for w in range(4):
+                a = rng.normal(size=8) + 1j * rng.normal(size=8)
+                c = C @ (a / np.linalg.norm(a))
+                s.set_initial_state(w, c, c)
+                cs.append(c)
+            if i in check:
+                kept[i] = (st, b, cs)
+            systems.append(s)
+        batch = hip.Batch(ctx, systems)
+        for _ in range(10):
+            batch.propagate(0.0, DT)
+        _say(f"  batch kernel: {batch.tuning()}")
+        for i, (st, b, cs) in kept.items():
+            S = O.overlap(st.coords, b)
+            h = O.huckel(b, S)
+            E, C = sl.eigh(h, S)
+            for w, c in enumerate(cs):
+                ke = C @ (np.exp(-1j * E * 10 * DT / O.H_BAR) * (C.T @ (S @ c)))
+                _, k = systems[i].wavepacket(w)
+                assert np.abs(k - ke).max() <= 1e-9 * np.abs(ke).max(), (i, w)
+                _, pops, _ = systems[i].observables(w)
+                pe = O.populations(b.fragment, 2, np.conj(ke), S @ ke)
+                assert np.abs(pops - pe).max() < 1e-10, (i, w)
+        batch.close()
+        for s in systems:
+            s.close()
+
+
+def test_batch_with_mixed_precision_member():
+    """A mixed-precision member runs its own FP32 matrix-core terms inside the
+    batch step: its state follows the same member stepped alone in the same
+    mode, and stays within 1e-6 in populations of the FP64 exact propagator."""
+    sizes = [300, 517, 640]
+    data = [_system_inputs(700 + i, n, 2) for i, n in enumerate(sizes)]
+    _say(f"batch with a mixed member, N={sizes}")
+    with hip.Context(0) as ctx:
+        batched, alone = [], None
+        for j, (st, b, S, E, C, kets) in enumerate(data):
+            s = hip.System(ctx, b, n_wp=2, n_frag=2)
+            if j == 2:
+                s.set_precision(hip.DH_PREC_MIXED)
+            s.build(st.coords)
+            for w, k in enumerate(kets):
+                s.set_wavepacket(w, S @ k, k)
+            batched.append(s)
+        st, b, S, E, C, kets = data[2]
+        alone = hip.System(ctx, b, n_wp=2, n_frag=2)
+        alone.set_precision(hip.DH_PREC_MIXED)
+        alone.build(st.coords)
+        for w, k in enumerate(kets):
+            alone.set_wavepacket(w, S @ k, k)
+        batch = hip.Batch(ctx, batched)
+        for _ in range(20):
+            batch.propagate(0.0, DT)
+            alone.propagate(0.0, DT)
+        for w in range(2):
+            bb, kb = batched[2].wavepacket(w)
+            ba, ka = alone.wavepacket(w)
+            assert np.abs(kb - ka).max() <= 1e-11 * np.abs(ka).max()
+            ke = C @ (np.exp(-1j * E * 20 * DT / O.H_BAR) * (C.T @ (S @ kets[w])))
+            pe = O.populations(b.fragment, 2, np.conj(ke), S @ ke)
+            _, pb, _ = batched[2].observables(w)
+            assert np.abs(pb - pe).max() <= 1e-6
+        # the FP64 members against their exact propagators
+        for j in (0, 1):
+            st, b, S, E, C, kets = data[j]
+            for w in range(2):
+                ke = C @ (np.exp(-1j * E * 20 * DT / O.H_BAR) * (C.T @ (S @ kets[w])))
+                _, kb = batched[j].wavepacket(w)
+                assert np.abs(kb - ke).max() <= 1e-9 * np.abs(ke).max()
+        batch.close()

@@ -95,3 +95,11 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_mixed_footprint_counts_fp32_copies():
+    """Serving admission counts the FP32 operator copies of a mixed-precision system."""
+    from dynemol_amd import hip
+    ld = 448
+    assert hip.system_bytes(400, 2, hip.DH_PREC_MIXED) - hip.system_bytes(400, 2) == 4 * ld * ld + 2 * ld * 4 * 2 * 4
+    assert hip.system_bytes(400, 2, hip.DH_PREC_F32_STORAGE) - hip.system_bytes(400, 2) == 4 * ld * ld
