@@ -193,7 +193,8 @@ def half_storage_bytes(N, R):
 
 
 def run_batched(st, b, S, n_wp, steps, seed=11):
-    """n_wp wavepackets on the N = 4096 system through the multi-RHS term kernel, FP64 and mixed."""
+    """n_wp wavepackets on the N = 4096 system through the multi-RHS term kernel:
+    FP64, mixed (FP32 products) and mixed with the residual correction."""
     from dynemol_amd import hip
     rng = np.random.default_rng(seed)
     K = rng.normal(size=(b.N, n_wp)) + 1j * rng.normal(size=(b.N, n_wp))
@@ -202,31 +203,32 @@ def run_batched(st, b, S, n_wp, steps, seed=11):
     kets = [K[:, w] / nrm[w] for w in range(n_wp)]
     bras = [SK[:, w] / nrm[w] for w in range(n_wp)]
     out = {}
+    modes = (hip.DH_PREC_FP64, hip.DH_PREC_MIXED, hip.DH_PREC_MIXED_CORRECTED)
     with hip.Context(0) as ctx:
-        for mixed in (False, True):
+        for mode in modes:
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
-            s.set_precision(mixed)
+            s.set_precision(mode)
             s.build(st.coords)
             for w, k in enumerate(kets):
                 s.set_wavepacket(w, bras[w], k)
-            t = 0.0
-            s.propagate(t, t + DT_PS)  # warm-up step (also autotunes)
-            t += DT_PS
+            s.propagate_steps(1, DT_PS)  # warm-up step (also autotunes)
             t0 = time.perf_counter()
-            for _ in range(steps):
-                s.propagate(t, t + DT_PS)
-                t += DT_PS
+            s.propagate_steps(steps, DT_PS)
             el = time.perf_counter() - t0
             pops = np.array([s.observables(w)[1] for w in range(n_wp)])
-            out[mixed] = {"value": n_wp * steps / el, "pops": pops, "tuning": s.tuning()["term"],
-                          "terms": s.stats()["terms"]}
+            out[mode] = {"value": n_wp * steps / el, "pops": pops, "tuning": s.tuning()["term"],
+                         "terms": s.stats()["terms"]}
             s.close()
-    err = float(np.abs(out[True]["pops"] - out[False]["pops"]).max())
-    log(f"batched: fp64 {out[False]['value']:.1f}, mixed {out[True]['value']:.1f} wavepacket-steps/s")
+    f = out[hip.DH_PREC_FP64]
+    m1, m3 = out[hip.DH_PREC_MIXED], out[hip.DH_PREC_MIXED_CORRECTED]
+    log(f"batched: fp64 {f['value']:.1f}, mixed {m1['value']:.1f}, corrected {m3['value']:.1f} wavepacket-steps/s")
     return {"n_orbitals": b.N, "wavepackets": n_wp, "steps": steps + 1,
-            "fp64_value": round(out[False]["value"], 3), "mixed_value": round(out[True]["value"], 3),
-            "fp64_term_variant": out[False]["tuning"], "chebyshev_terms": out[False]["terms"],
-            "mixed_pop_err_vs_fp64": err, "unit": "wavepacket-steps/s"}
+            "fp64_value": round(f["value"], 3), "mixed_value": round(m1["value"], 3),
+            "mixed_corrected_value": round(m3["value"], 3),
+            "fp64_term_variant": f["tuning"], "chebyshev_terms": f["terms"],
+            "mixed_pop_err_vs_fp64": float(np.abs(m1["pops"] - f["pops"]).max()),
+            "mixed_corrected_pop_err_vs_fp64": float(np.abs(m3["pops"] - f["pops"]).max()),
+            "unit": "wavepacket-steps/s"}
 
 
 def run_serve(n_req, lam, seed=5, n_lo=256, n_hi=8192, budget_gib=96, check=True):
@@ -409,7 +411,8 @@ def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100, system="in
            "steps": steps}
     with hip.Context(0) as ctx:
         res = {}
-        names = {hip.DH_PREC_FP64: "fp64", hip.DH_PREC_MIXED: "mixed", hip.DH_PREC_F32_STORAGE: "f32_storage"}
+        names = {hip.DH_PREC_FP64: "fp64", hip.DH_PREC_MIXED_CORRECTED: "mixed_corrected",
+                 hip.DH_PREC_MIXED: "mixed", hip.DH_PREC_F32_STORAGE: "f32_storage"}
         for mode, name in names.items():
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=len(st.fragment_names))
             s.set_precision(mode)
@@ -426,17 +429,14 @@ def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100, system="in
                 log("precision study: initial states ready")
             for w in range(n_wp):
                 s.set_wavepacket(w, bras[w], kets[w])
-            t, el, hist = 0.0, 0.0, []
-            for step in range(steps):
+            el, hist = 0.0, []
+            for block in range(steps // check_every):
                 t0 = time.perf_counter()
-                s.propagate(t, t + DT_PS)
+                s.propagate_steps(check_every, DT_PS)
                 el += time.perf_counter() - t0
-                t += DT_PS
-                if (step + 1) % 10 == 0:
-                    log(f"precision study {name}: step {step + 1}, {s.stats()['terms']} terms")
-                if (step + 1) % check_every == 0:
-                    obs = [s.observables(w) for w in range(n_wp)]
-                    hist.append((np.array([o[1] for o in obs]), np.array([o[2] for o in obs])))
+                log(f"precision study {name}: step {(block + 1) * check_every}, {s.stats()['terms']} terms")
+                obs = [s.observables(w) for w in range(n_wp)]
+                hist.append((np.array([o[1] for o in obs]), np.array([o[2] for o in obs])))
             res[name] = {"hist": hist, "value": n_wp * steps / el, "terms": s.stats()["terms"],
                          "bounds": s.bounds()[:2], "drift": s.stats()["norm_drift"]}
             s.close()
@@ -444,7 +444,7 @@ def run_precision_study(steps=1000, seed=4, n_wp=64, check_every=100, system="in
     out["chebyshev_terms"] = f["terms"]
     out["fp64_wavepacket_steps_per_s"] = round(f["value"], 2)
     out["fp64_norm_drift_last_step"] = f["drift"]
-    for name, key in (("mixed", ""), ("f32_storage", "f32s_")):
+    for name, key in (("mixed_corrected", "corr_"), ("mixed", ""), ("f32_storage", "f32s_")):
         m = res[name]
         out[f"{name}_wavepacket_steps_per_s"] = round(m["value"], 2)
         out[f"{key}pop_err_by_step"] = {str(check_every * (i + 1)): float(np.abs(pm - pf).max())
@@ -657,6 +657,7 @@ def main():
         out["batched"] = run_batched(st, b, S, args.batched_wp, args.batched_steps)
         out["batched"]["fp64_vs_dropin"] = round(out["batched"]["fp64_value"] / value, 3)
         out["batched"]["mixed_vs_dropin"] = round(out["batched"]["mixed_value"] / value, 3)
+        out["batched"]["mixed_corrected_vs_dropin"] = round(out["batched"]["mixed_corrected_value"] / value, 3)
         if args.many_steps > 0:
             log("many-systems leg")
             out["many_systems"] = run_many_systems(steps=args.many_steps)

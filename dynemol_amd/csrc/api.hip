@@ -130,9 +130,15 @@ struct dh_system {
     // each T_k as its epilogue writes it), alternating between two buffers
     float* x32[2] = {nullptr, nullptr};
     bool h32_ready = false;
+    unsigned short* Hlo = nullptr;  // mode 3: bf16 residual H_orth - H32 (the GEMM's 16-k block layout)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> evpool;  // per-step series events of dh_propagate_steps (timing on)
 };
+
+// modes whose term products are FP32 (1, and 3 with its residual correction):
+// their FP32 rounding drifts the norm ~1e-7 per step, which a per-step FP64
+// norm restoration removes
+static bool fp32_products(const dh_system* s) { return s->precision == 1 || s->precision == 3; }
 
 #define CHECK_ARG(cond, msg)                  \
     do {                                      \
@@ -211,6 +217,7 @@ extern "C" int dh_system_destroy(dh_system* s) {
     hipSetDevice(s->ctx->device);
     hipStreamSynchronize(s->ctx->stream);
     if (s->H32) hipFree(s->H32);
+    if (s->Hlo) hipFree(s->Hlo);
     for (float* p : s->x32)
         if (p) hipFree(p);
     for (auto& g : s->graphs)
@@ -266,8 +273,9 @@ extern "C" long long dh_system_bytes(int n_orb, int n_wp) {
 extern "C" long long dh_system_bytes_prec(int n_orb, int n_wp, int precision) {
     const long long ld = round_ld(n_orb);
     long long b = dh_system_bytes(n_orb, n_wp);
-    if (precision != 0) b += ld * ld * 4;            // H32
-    if (precision == 1) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
+    if (precision != 0) b += ld * ld * 4;                       // H32
+    if (precision == 3) b += ld * ld * 2;                       // bf16 residual
+    if (precision == 1 || precision == 3) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
     return b;
 }
 
@@ -650,7 +658,8 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
     }
     s->h32_ready = false;
     if (s->precision != 0) {
-        launch_to_f32(s->Hth, s->H32, (long)ld * ld, st);
+        if (s->precision == 3) launch_split_f32_bf16(s->Hth, s->H32, s->Hlo, ld, st);
+        else launch_to_f32(s->Hth, s->H32, (long)ld * ld, st);
         s->h32_ready = true;
     }
     int rc = compute_bounds(s);
@@ -833,7 +842,8 @@ static int ensure_coefficients(dh_system* s, double dt) {
 // the recurrence alternates between x0 and x1.
 struct SeriesOps {
     const double* H;
-    const float* H32;      // precision modes 1 and 2
+    const float* H32;      // precision modes 1, 2 and 3
+    const void* Mlo;       // precision mode 3
     int ld, N, R;
     const cplx* phi;
     cplx *x0, *x1, *acc, *partial;
@@ -861,16 +871,18 @@ static void series_core(const SeriesOps& o, hipStream_t st) {
     a.acc = o.acc;
     a.coef = o.coef;
     a.M32 = o.H32;
+    a.Mlo = o.Mlo;
     const int prec = o.prec;
     // mode 1: the FP32 GEMM reads FP32 copies of its input; the step's state
     // is converted once, then each term's epilogue writes the next one's
-    if (prec == 1) launch_to_f32(reinterpret_cast<const double*>(o.phi), o.x32[0], (long)o.ld * 2 * R, st);
+    const bool f32x = prec == 1 || prec == 3;  // FP32 products on FP32 copies of x
+    if (f32x) launch_to_f32(reinterpret_cast<const double*>(o.phi), o.x32[0], (long)o.ld * 2 * R, st);
     auto term = [&](MatvecArgs args) {
-        if (prec == 1) {
+        if (f32x) {
             args.x32 = o.x32[(args.k - 1) & 1];
             args.out32 = o.x32[args.k & 1];
         }
-        if (prec == 1) launch_gemm3(args, 1, reinterpret_cast<double*>(o.partial), st);
+        if (f32x) launch_gemm3(args, prec, reinterpret_cast<double*>(o.partial), st);
         else if (prec == 2) launch_term_f32h(args, st);
         else launch_matvec(args, st);
     };
@@ -904,6 +916,7 @@ static int series_launches(dh_system* s) {
     SeriesOps o{};
     o.H = s->Hth;
     o.H32 = s->H32;
+    o.Mlo = s->Hlo;
     o.ld = s->ld;
     o.N = s->N;
     o.R = 2 * s->W;
@@ -1011,7 +1024,7 @@ static int propagate_checked(dh_system* s, double dt, int* n_terms) {
     // FP64 norm drift of one step that signals bounds not enclosing the
     // spectrum; the mixed mode's FP32 products drift ~1e-7 per step by themselves
     // (mode 2 propagates with the exactly symmetric FP32-rounded H: FP64 drift)
-    const double drift_tol = s->precision == 1 ? 1e-4 : 1e-9;
+    const double drift_tol = fp32_products(s) ? 1e-4 : 1e-9;
     for (int attempt = 0; attempt < 4; ++attempt) {
         int rc = ensure_coefficients(s, dt);
         if (rc) return rc;
@@ -1023,7 +1036,7 @@ static int propagate_checked(dh_system* s, double dt, int* n_terms) {
         launch_column_norms(s->N, R, s->phi, s->norms1, st);
         // the outputs go out with the norms, so one host wait covers the step;
         // an attempt that fails the drift check below recomputes them
-        if (s->precision == 1) launch_renorm_columns(s->phi, s->N, R, s->norms0, s->norms1, st);
+        if (fp32_products(s)) launch_renorm_columns(s->phi, s->N, R, s->norms0, s->norms1, st);
         launch_outputs(s);
         DH_HIP(hipMemcpyAsync(n0.data(), s->norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         DH_HIP(hipMemcpyAsync(n1.data(), s->norms1, sizeof(double) * R, hipMemcpyDeviceToHost, st));
@@ -1150,7 +1163,7 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
         DH_HIP(hipMemcpyAsync(d.geo, coords, sizeof(double) * geo_n * n_geo, hipMemcpyHostToDevice, st));
     }
     std::vector<double> h0((size_t)n_steps * R), h1((size_t)n_steps * R);
-    const double drift_tol = s->precision == 1 ? 1e-4 : 1e-9;
+    const double drift_tol = fp32_products(s) ? 1e-4 : 1e-9;
     int last_terms = 0;
     for (int j = 0; j < n_steps;) {
         if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) {
@@ -1181,7 +1194,7 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
             if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * (i - j) + 1], st));
             std::swap(s->phi, s->acc);
             launch_column_norms(s->N, R, s->phi, d.n1 + (size_t)i * R, st);
-            if (s->precision == 1)
+            if (fp32_products(s))
                 launch_renorm_columns(s->phi, s->N, R, d.n0 + (size_t)i * R, d.n1 + (size_t)i * R, st);
             launch_outputs(s);
             DH_HIP(hipMemcpyAsync(d.pops + (size_t)i * W * NP, s->pops, sizeof(double) * W * NP,
@@ -1694,16 +1707,20 @@ extern "C" int dh_test_matvec_timing(int N, int R, int variant, int reps, double
 }
 
 extern "C" int dh_set_precision(dh_system* s, int mode) {
-    CHECK_ARG(s && mode >= 0 && mode <= 2,
-              "dh_set_precision: mode must be 0 (FP64), 1 (mixed) or 2 (FP32-stored operator)");
+    CHECK_ARG(s && mode >= 0 && mode <= 3,
+              "dh_set_precision: mode must be 0 (FP64), 1 (mixed), 2 (FP32-stored operator) or 3 (mixed with the "
+              "residual correction)");
     DH_HIP(hipSetDevice(s->ctx->device));
     if (mode != 0 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
-    if (mode == 1)
+    if (mode == 3 && !s->Hlo) DALLOC(s->Hlo, (size_t)s->ld * s->ld);
+    if (mode == 1 || mode == 3)
         for (float*& p : s->x32)
             if (!p) DALLOC(p, (size_t)s->ld * 4 * s->W);
+    if (mode != s->precision) s->h32_ready = false;  // mode 3 also needs the residual
     s->precision = mode;
     if (mode != 0 && s->ops_ready && !s->h32_ready) {
-        launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);
+        if (mode == 3) launch_split_f32_bf16(s->Hth, s->H32, s->Hlo, s->ld, s->ctx->stream);
+        else launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);
         DH_HIP(hipStreamSynchronize(s->ctx->stream));
         s->h32_ready = true;
     }
@@ -2132,7 +2149,7 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     std::vector<int> ok(n, 1);
     double maxdrift = 0.0;
     for (int i = 0; i < n; ++i) {
-        const double tol = b->sys[i]->precision == 1 ? 1e-4 : 1e-9;
+        const double tol = fp32_products(b->sys[i]) ? 1e-4 : 1e-9;
         for (int c = 0; c < R; ++c) {
             const double n0 = nv[(size_t)i * R + c], n1 = nv[(size_t)n * R + (size_t)i * R + c];
             const double d = n0 > 0 ? std::fabs(n1 - n0) / n0 : 0.0;
@@ -2148,7 +2165,7 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     launch_grouped_copy(b->d_copy, (int)ct.size(), st);
     // the mixed mode's per-step FP64 norm restoration, as in its single step
     for (int i = 0; i < n; ++i)
-        if (ok[i] && b->sys[i]->precision == 1)
+        if (ok[i] && fp32_products(b->sys[i]))
             launch_renorm_columns(b->sys[i]->phi, b->sys[i]->N, R, b->d_nval + (size_t)i * R,
                                   b->d_nval + (size_t)n * R + (size_t)i * R, st);
     // outputs of every member

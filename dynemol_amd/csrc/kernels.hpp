@@ -135,6 +135,7 @@ struct MatvecArgs {
     const float* M32;       // FP32 copy of M (mixed and FP32-storage precision modes; may be null)
     const float* x32;       // mixed mode: FP32 copy of x (same layout), read by the FP32 GEMM; may be null
     float* out32;           // mixed mode: the epilogue also writes T_k here in FP32 (same layout); may be null
+    const void* Mlo;        // precision mode 3: bf16 residual M - fl32(M) in the GEMM's 16-k block layout
 };
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
@@ -184,6 +185,8 @@ void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* o
 void launch_column_norms(int N, int R, const cplx* x, double* out, hipStream_t st);
 void launch_renorm_columns(cplx* x, int N, int R, const double* n0, const double* n1, hipStream_t st);
 void launch_to_f32(const double* a, float* b, long n, hipStream_t st);
+// H32 = fl32(H) and the bf16 residual H - H32 in the PREC 3 GEMM's layout (2 ld^2 bytes)
+void launch_split_f32_bf16(const double* H, float* H32, void* Mlo, int ld, hipStream_t st);
 void launch_identity_pad(double* S, double* h, int N, int ld, hipStream_t st);
 
 

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DYNEMOL_HIP_LIB", os.path.join(HERE, "libdynemol_hip.so"))
 
 DH_MAT_S, DH_MAT_H, DH_MAT_L, DH_MAT_LINV, DH_MAT_HORTH, DH_MAT_HPRIME = range(6)
-DH_PREC_FP64, DH_PREC_MIXED, DH_PREC_F32_STORAGE = range(3)
+DH_PREC_FP64, DH_PREC_MIXED, DH_PREC_F32_STORAGE, DH_PREC_MIXED_CORRECTED = range(4)
 H_BAR = 6.58264e-4
 
 _lib = None
@@ -350,7 +350,8 @@ class System:
 
     def set_precision(self, mode):
         """Term arithmetic: 0 / False = FP64, 1 / True = mixed (FP32 products),
-        2 = FP32-stored H_orth with FP64 products (DH_PREC_* constants)."""
+        2 = FP32-stored H_orth with FP64 products, 3 = mixed with the FP64
+        residual correction (DH_PREC_* constants)."""
         m = int(mode)
         _check(lib().dh_set_precision(self.h, m), "dh_set_precision")
 

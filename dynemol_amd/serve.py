@@ -12,7 +12,8 @@ dt): what an MD driver hands the propagator each nuclear step
     batch_max_n and advances them together, grouped by padded wavepacket count
     into one grouped launch per Chebyshev term (small systems: launch overhead
     dominates).  The other lanes take the larger and the mixed-precision
-    requests, each stepping with the kernels the autotuner picked for its shape
+    requests (FP32 matrix-core products with the FP64 residual correction),
+    each stepping with the kernels the autotuner picked for its shape
     (symmetric half-storage term, matrix-core GEMM), several steps per call
     (dh_propagate_steps) round-robin over the lane's requests;
   * a lane builds S, h and the orthogonal operators of a request on the device
@@ -95,7 +96,7 @@ class _Lane:
         w = padded_wp(len(req.psi_ket))
         s = hip.System(self.ctx, req.basis, n_wp=w, n_frag=req.n_frag)
         if req.mixed:
-            s.set_precision(hip.DH_PREC_MIXED)
+            s.set_precision(hip.DH_PREC_MIXED_CORRECTED)
         s.build(req.coords)
         N = req.basis.N
         for i in range(w):
@@ -225,7 +226,7 @@ class Server:
         """Resident device bytes of the request's system (operators, state,
         buffers, and the FP32 operator copies of a mixed-precision request)."""
         return hip.system_bytes(req.basis.N, padded_wp(len(req.psi_ket)),
-                                hip.DH_PREC_MIXED if req.mixed else hip.DH_PREC_FP64)
+                                hip.DH_PREC_MIXED_CORRECTED if req.mixed else hip.DH_PREC_FP64)
 
     @staticmethod
     def workspace_bytes(n_max: int) -> int:

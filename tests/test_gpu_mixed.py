@@ -62,31 +62,48 @@ def test_gemm_term_many_wavepackets_exact(c1):
         hip.force_matvec_variant(-1)
 
 
-def test_mixed_precision_tracks_fp64(c1):
+def test_mixed_precision_holds_the_bar_over_1000_steps(c1):
+    """1000 steps of 0.5 fs, 4 wavepackets: the corrected mixed mode (FP32
+    products + the bf16-stored FP64 residual of the FP32 operator) stays within
+    the 1e-6 population bar of FP64; the plain mixed mode is reported beside it
+    and must at least hold 1e-5.  (On this 416-orbital system the random FP32
+    rounding, not the rounding of the operator, sets both errors, so the
+    correction's gain shows on the 8192-orbital workload instead.)  The pre-renormalisation norm drift of one step
+    stays at FP32 level (the renormalisation cannot hide a larger error)."""
     st, b, S, h, E, C = c1
-    n_wp, nsteps = 4, 20
-    _say(f"mixed vs FP64, {n_wp} wavepackets, {nsteps} steps")
+    n_wp, nsteps = 4, 1000
+    _say(f"mixed modes vs FP64, {n_wp} wavepackets, {nsteps} steps")
     kets = _states(C, S, n_wp, 2)
     out = {}
     with hip.Context(0) as ctx:
-        for mixed in (False, True):
+        for mode in (hip.DH_PREC_FP64, hip.DH_PREC_MIXED, hip.DH_PREC_MIXED_CORRECTED):
             s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
-            s.set_precision(mixed)
+            s.set_precision(mode)
             s.build(st.coords)
             for w, k in enumerate(kets):
                 s.set_wavepacket(w, S @ k, k)
-            t = 0.0
-            for _ in range(nsteps):
-                s.propagate(t, t + DT)
-                t += DT
-            out[mixed] = [(s.wavepacket(w), s.observables(w)[1]) for w in range(n_wp)]
+            pops, erg, _ = s.propagate_steps(nsteps, DT)
+            out[mode] = (pops, [s.wavepacket(w) for w in range(n_wp)], s.stats()["norm_drift"])
             s.close()
-    for w in range(n_wp):
-        (b64, k64), p64 = out[False][w]
-        (b32, k32), p32 = out[True][w]
-        assert np.abs(k32 - k64).max() < 1e-4 * np.abs(k64).max()
-        assert np.abs(p32 - p64).max() < 1e-4
-        assert np.abs(p32 - p64).max() > 0.0  # the mixed path really ran in FP32
+    p64 = out[hip.DH_PREC_FP64][0]
+    err1 = np.abs(out[hip.DH_PREC_MIXED][0] - p64).max(axis=(1, 2))
+    err3 = np.abs(out[hip.DH_PREC_MIXED_CORRECTED][0] - p64).max(axis=(1, 2))
+    _say(f"  max population difference after 100/1000 steps: mixed {err1[99]:.2e}/{err1[-1]:.2e}, "
+         f"corrected {err3[99]:.2e}/{err3[-1]:.2e}; last-step drift mixed {out[1][2]:.1e}, corrected {out[3][2]:.1e}")
+    assert err3[-1] <= 1e-6
+    assert err1[-1] <= 1e-5
+    assert err3[-1] > 0.0 and err1[-1] > 0.0  # the FP32 paths really ran
+    assert out[hip.DH_PREC_MIXED][2] < 1e-5 and out[hip.DH_PREC_MIXED_CORRECTED][2] < 1e-5
+    # the final kets against the exact propagator: FP64 at its bar; the FP32
+    # paths carry a phase error that grows with time (populations do not see it)
+    for w, k in enumerate(kets):
+        ke = C @ (np.exp(-1j * E * nsteps * DT / O.H_BAR) * (C.T @ (S @ k)))
+        _, k64 = out[hip.DH_PREC_FP64][1][w]
+        _, k3 = out[hip.DH_PREC_MIXED_CORRECTED][1][w]
+        assert np.abs(k64 - ke).max() <= 1e-9 * np.abs(ke).max()
+        e3 = np.abs(k3 - ke).max() / np.abs(ke).max()
+        _say(f"  wavepacket {w}: corrected mixed |dPsi|/|Psi| = {e3:.1e} after {nsteps} steps")
+        assert e3 <= 1e-3
 
 
 @pytest.mark.parametrize("n_wp", [2, 20])
