@@ -1755,12 +1755,13 @@ struct dh_batch {
     hipGraphExec_t gexec = nullptr;
     std::vector<double> gkey;
     // grouped term kernel: symmetric half-storage (R <= 8; phase-1 tasks:
-    // segments, phase-2 tasks: block x chunk, each system with the segment
-    // length its own single launch would use) or dense column blocks, as the
-    // batch autotuner chose for this R
+    // segments of up to 8 tiles, phase-2 tasks: block x chunk; both shapes a
+    // function of the system's ld alone) or dense column blocks, as the batch
+    // autotuner chose for this R
     bool sym_ok = false, sym = false;
     int *d_t1_desc = nullptr, *d_t1_idx = nullptr, *d_t2_desc = nullptr, *d_t2_idx = nullptr, *d_sysS = nullptr;
     std::vector<int> prefix1, prefix2;
+    int reduce_groups = 4;  // phase-2 groups: those of the members' own launches (the largest)
     // members in a precision mode other than FP64 run their own term series
     // (FP32 matrix-core GEMM or FP32-stored operator) inside the batch step
     std::vector<char> solo;
@@ -1784,7 +1785,7 @@ static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k
     const int n = b->n;
     if (sym)
         launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
-                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, st);
+                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, b->reduce_groups, st);
     else
         launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], b->R, k,
                               st);
@@ -1914,6 +1915,7 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
         for (int i = 0; i < n; ++i) {
             t1 += sym_term_segments(b->sys[i]->ld, &sS[i]);
             t2 += (long)(b->sys[i]->ld / TILE) * R;
+            b->reduce_groups = std::max(b->reduce_groups, sym_reduce_groups_for(b->sys[i]->ld));
         }
         BALLOC(b->d_t1_desc, (size_t)t1);
         BALLOC(b->d_t1_idx, (size_t)t1);

@@ -1281,23 +1281,28 @@ __global__ __launch_bounds__(256) void sym_tiles_kernel(const double* __restrict
 }
 
 // Phase 2: workgroup (B, c) owns 64 consecutive complex outputs of block B
-// (rows B*64 + (c*64 + e) / R, column (c*64 + e) % R).  16 groups of 64
+// (rows B*64 + (c*64 + e) / R, column (c*64 + e) % R).  G groups of 64
 // threads split the source list; each group issues up to SYM_MAXS loads
 // before adding them (latency, not bandwidth, bounds this kernel), and the
-// group sums are added in group order.
+// group sums are added in group order.  G follows the block count: a block
+// has about nb sources, so G = 16 spreads the 64 sources of nb = 64 four per
+// thread, and G = 4 keeps a small system's workgroup busy (G = 16 left most
+// of its groups without a source).  The order of additions depends on (nb, G)
+// only, so a system gives the same bits alone and in a batch.
 constexpr int SYM_G = 16, SYM_MAXS = 8;
 constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per source group
 static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
+__host__ __device__ inline int sym_reduce_groups(int nb) { return nb > 32 ? 16 : 4; }
 
-template <int R>
+template <int R, int G>
 __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int chunk) {
-    __shared__ cplx red[SYM_G * 64];
+    __shared__ cplx red[G * 64];
     const int nb = a.ld / TILE;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
     const int seg0 = sym_segment_start(B, S);
     const int nseg = sym_segments_in_row(B, S);
     const int L = nseg + (nb - 1 - B);   // sources: row segments, then Pcol(I, B), I > B
-    const int s0 = g * L / SYM_G, s1 = (g + 1) * L / SYM_G;
+    const int s0 = g * L / G, s1 = (g + 1) * L / G;
     const long E = (long)TILE * R;
     const long pcol = (long)nb * (nb - 1) / 2 * E;
     // the epilogue's operands do not depend on the partials: fetch them first
@@ -1328,7 +1333,7 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     if (t >= 64) return;
     cplx y{0.0, 0.0};
 #pragma unroll
-    for (int gg = 0; gg < SYM_G; ++gg) {
+    for (int gg = 0; gg < G; ++gg) {
         y.re += red[gg * 64 + t].re;
         y.im += red[gg * 64 + t].im;
     }
@@ -1336,22 +1341,30 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
 }
 
+template <int R, int G>
+__global__ __launch_bounds__(G * 64) void sym_reduce_kernel(MatvecArgs a, int S) {
+    sym_reduce_body<R, G>(a, S, blockIdx.x, blockIdx.y);
+}
+
 template <int R>
-__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_kernel(MatvecArgs a, int S) {
-    sym_reduce_body<R>(a, S, blockIdx.x, blockIdx.y);
+static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st) {
+    if (sym_reduce_groups(nb) == 16)
+        hipLaunchKernelGGL((sym_reduce_kernel<R, 16>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
+    else
+        hipLaunchKernelGGL((sym_reduce_kernel<R, 4>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
 }
 
 // Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),
 // with the system's segment length sys_S[task_desc[i]]
-template <int R>
-__global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym_reduce_grouped_kernel(const MatvecArgs* __restrict__ args,
-                                                                                const int* __restrict__ task_desc,
-                                                                                const int* __restrict__ task_idx,
-                                                                                const int* __restrict__ sys_S, int k) {
+template <int R, int G>
+__global__ __launch_bounds__(G * 64) void sym_reduce_grouped_kernel(const MatvecArgs* __restrict__ args,
+                                                                    const int* __restrict__ task_desc,
+                                                                    const int* __restrict__ task_idx,
+                                                                    const int* __restrict__ sys_S, int k) {
     const int d = task_desc[blockIdx.x], v = task_idx[blockIdx.x];
     MatvecArgs a = args[d];
     a.k = k;
-    sym_reduce_body<R>(a, sys_S[d], v / R, v % R);
+    sym_reduce_body<R, G>(a, sys_S[d], v / R, v % R);
 }
 
 template <int R>
@@ -1360,7 +1373,7 @@ static void launch_sym(const MatvecArgs& a, hipStream_t st) {
     const int S = sym_seg_for(nb);
     hipLaunchKernelGGL((sym_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M, a.x,
                        a.partial, a.ld, a.N, a.x_stride, a.x_off, S);
-    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
+    launch_sym_reduce<R>(a, nb, S, st);
 }
 
 // ------------------------------------------------------------------ symmetric, matrix-core tiles
@@ -1541,16 +1554,20 @@ __global__ __launch_bounds__(256) void symm_tiles_grouped_kernel(const MatvecArg
                                   a.ld, a.N, a.x_stride, a.x_off, sys_S[d], task_idx[blockIdx.x]);
 }
 
-// segment length S and segment count of the 4x4x4 symmetric term at ld (the
-// grid of launch_symm below)
+// Segment length S and segment count of a system's symmetric term in a batch:
+// up to 8 tiles of a block row per workgroup.  A batch has many systems to fill
+// the device, so it takes long segments (row sums stay in registers across the
+// segment: fewer partial sums to write and to add) where a lone system
+// shortens them to fill 256 CUs.  S depends on ld only, so a system gives the
+// same bits in every batch.
 int sym_term_segments(int ld, int* S) {
     const int nb = ld / TILE;
-    *S = sym_seg_for(nb, 3);
+    *S = nb < 8 ? nb : 8;
     return sym_segment_count(nb, *S);
 }
 
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, hipStream_t st) {
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st) {
     if (n1 > 0) switch (R) {
             case 1: hipLaunchKernelGGL(symm_tiles_grouped_kernel<1>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
             case 2: hipLaunchKernelGGL(symm_tiles_grouped_kernel<2>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
@@ -1558,14 +1575,28 @@ void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int*
             case 8: hipLaunchKernelGGL(symm_tiles_grouped_kernel<8>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
             default: break;
         }
-    if (n2 > 0) switch (R) {
-            case 1: hipLaunchKernelGGL(sym_reduce_grouped_kernel<1>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
-            case 2: hipLaunchKernelGGL(sym_reduce_grouped_kernel<2>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
-            case 4: hipLaunchKernelGGL(sym_reduce_grouped_kernel<4>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
-            case 8: hipLaunchKernelGGL(sym_reduce_grouped_kernel<8>, dim3(n2), dim3(SYM_REDUCE_THREADS), 0, st, d_args, t2_desc, t2_idx, sys_S, k); break;
+    if (n2 <= 0) return;
+#define DH_SYM_RED(RR, GG)                                                                                      \
+    hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG>), dim3(n2), dim3(GG * 64), 0, st, d_args, t2_desc, t2_idx, \
+                       sys_S, k)
+    if (reduce_groups == 16) switch (R) {
+            case 1: DH_SYM_RED(1, 16); break;
+            case 2: DH_SYM_RED(2, 16); break;
+            case 4: DH_SYM_RED(4, 16); break;
+            case 8: DH_SYM_RED(8, 16); break;
             default: break;
         }
+    else switch (R) {
+            case 1: DH_SYM_RED(1, 4); break;
+            case 2: DH_SYM_RED(2, 4); break;
+            case 4: DH_SYM_RED(4, 4); break;
+            case 8: DH_SYM_RED(8, 4); break;
+            default: break;
+        }
+#undef DH_SYM_RED
 }
+
+int sym_reduce_groups_for(int ld) { return sym_reduce_groups(ld / TILE); }
 
 // MT = float: H's FP32 copy (a.M32), widened to FP64 on load (precision mode 2)
 template <int R, int MF, typename MT = double>
@@ -1578,7 +1609,7 @@ static void launch_symm(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((symm_tiles_kernel<R, MF, MT>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off, S);
-    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
+    launch_sym_reduce<R>(a, nb, S, st);
 }
 
 // ------------------------------------------------------------------ symmetric, column strips
@@ -1726,7 +1757,7 @@ static void launch_symm3(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((symm3_tiles_kernel<R>), dim3(sym_segment_count(nb, S)), dim3(256), 0, st, a.M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off, S);
-    hipLaunchKernelGGL((sym_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a, S);
+    launch_sym_reduce<R>(a, nb, S, st);
 }
 
 // ------------------------------------------------------------------ symmetric, 2 x 2 super-tiles

@@ -165,10 +165,13 @@ void launch_matvec_grouped(const MatvecArgs* d_args, const int* d_task_desc, con
 // Symmetric half-storage term (4x4x4 products, R in {1, 2, 4, 8}) over many
 // systems: phase-1 task i is segment t1_idx[i] of system t1_desc[i], phase-2
 // task i is block * R + chunk = t2_idx[i] of system t2_desc[i]; sys_S[d] is
-// system d's segment length, as sym_term_segments gives it.
+// system d's segment length in a batch, as sym_term_segments gives it.
 int sym_term_segments(int ld, int* S);
+// reduce_groups: the phase-2 group count of the batch's systems (sym_reduce_groups_for
+// their ld; a batch whose systems differ in it uses the largest)
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, hipStream_t st);
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st);
+int sym_reduce_groups_for(int ld);
 
 void launch_zero_u32(unsigned* p, int n, hipStream_t st);
 void launch_zero_cplx(cplx* p, long n, hipStream_t st);
