@@ -71,6 +71,13 @@ struct dh_context {
     const dh_system* ws_owner = nullptr;  // system whose S and h the workspace holds
     dh_system* dropin = nullptr;
     int dropin_N = 0;
+    // operator builds overlapped with propagation (dh_propagate_steps): a
+    // second stream and a second workspace of ws_ld, swapped with the first
+    // when the build is taken over
+    hipStream_t side = nullptr;
+    double *S2 = nullptr, *h2 = nullptr, *W2 = nullptr, *Hp2 = nullptr, *Dinv2 = nullptr;
+    int* info2 = nullptr;
+    int ws2_ld = 0;
 };
 
 struct dh_system {
@@ -117,6 +124,7 @@ struct dh_system {
         hipGraphExec_t exec = nullptr;
         int K = -1, prec = -1, variant = -1;
         const void* phi = nullptr;
+        const void* H = nullptr;  // the operator buffer (swapped by overlapped rebuilds)
         double c = 0.0, r = 0.0;
     } graphs[2];
     int next_graph = 0;
@@ -133,6 +141,18 @@ struct dh_system {
     unsigned short* Hlo = nullptr;  // mode 3: bf16 residual H_orth - H32 (the GEMM's 16-k block layout)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> evpool;  // per-step series events of dh_propagate_steps (timing on)
+    // the next geometry's operators, built on the context's side stream while
+    // the current interval's steps run (dh_propagate_steps), then swapped in
+    struct Shadow {
+        double *Q1 = nullptr, *Q2 = nullptr, *Hth = nullptr, *coords = nullptr;
+        float* H32 = nullptr;
+        unsigned short* Hlo = nullptr;
+        cplx *lv = nullptr, *lvp = nullptr, *lw = nullptr, *partial = nullptr;
+        double *lal = nullptr, *lbe = nullptr;
+        unsigned* counters = nullptr;
+        double bounds[6] = {0, 0, 0, 0, 0, 0};
+        bool allocated = false;
+    } sh;
 };
 
 // modes whose term products are FP32 (1, and 3 with its residual correction):
@@ -189,12 +209,23 @@ extern "C" int dh_init(int device, dh_context** out) {
     return DH_OK;
 }
 
+static void free_ws2(dh_context* c) {
+    if (c->side) hipStreamSynchronize(c->side);
+    for (double* p : {c->S2, c->h2, c->W2, c->Hp2, c->Dinv2})
+        if (p) hipFree(p);
+    if (c->info2) hipFree(c->info2);
+    c->S2 = c->h2 = c->W2 = c->Hp2 = c->Dinv2 = nullptr;
+    c->info2 = nullptr;
+    c->ws2_ld = 0;
+}
+
 static void free_ws(dh_context* c) {
     hipFree(c->S); hipFree(c->h); hipFree(c->W); hipFree(c->Hp); hipFree(c->Dinv);
     if (c->Kd) hipFree(c->Kd);
     c->S = c->h = c->W = c->Hp = c->Dinv = c->Kd = nullptr;
     c->ws_ld = 0;
     c->ws_owner = nullptr;
+    free_ws2(c);
 }
 
 static int ensure_ws(dh_context* c, int ld) {
@@ -225,6 +256,10 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
+    void* shp[] = {s->sh.Q1, s->sh.Q2, s->sh.Hth, s->sh.coords, s->sh.H32, s->sh.Hlo, s->sh.lv, s->sh.lvp, s->sh.lw,
+                   s->sh.partial, s->sh.lal, s->sh.lbe, s->sh.counters};
+    for (void* p : shp)
+        if (p) hipFree(p);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out, s->stage,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
@@ -242,6 +277,7 @@ extern "C" int dh_finalize(dh_context* c) {
     free_ws(c);
     lib_destroy(c->lib);
     hipFree(c->d_info);
+    if (c->side) hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
     return DH_OK;
@@ -490,36 +526,72 @@ static void convert_ao_to_phi(dh_system* s) {
     launch_matvec(a, st);
 }
 
-static int compute_bounds(dh_system* s) {
-    hipStream_t st = s->ctx->stream;
+// Buffers one operator build writes and the stream it runs on: the system's own
+// operators with the context workspace (a build in place), or the shadow set
+// that a build overlapped with propagation fills on the side stream.
+struct OpsTarget {
+    hipStream_t st;
+    double *S, *h, *W, *Dinv;
+    int* info;
+    double *Q1, *Q2, *Hth;
+    float* H32;
+    unsigned short* Hlo;
+    cplx *lv, *lvp, *lw, *partial;
+    double *lal, *lbe;
+    unsigned* counters;
+    double* bounds;  // 6 values
+};
+
+static OpsTarget main_target(dh_system* s) {
+    dh_context* c = s->ctx;
+    return OpsTarget{c->stream, c->S, c->h, c->W, c->Dinv, c->d_info, s->Q1, s->Q2, s->Hth, s->H32, s->Hlo,
+                     s->lv, s->lvp, s->lw, s->partial, s->lal, s->lbe, s->counters, s->bounds_h};
+}
+
+// Lanczos spectral bounds of t.Hth into t.bounds (host wait on t.st)
+static int bounds_into(dh_system* s, const OpsTarget& t) {
+    hipStream_t st = t.st;
     const int m = std::min(s->lanczos_steps, std::min(LANCZOS_MAX, s->N));
-    launch_lanczos_start(s->N, s->lv, s->lvp, 0x9e3779b9u, st);
+    launch_lanczos_start(s->N, t.lv, t.lvp, 0x9e3779b9u, st);
     MatvecArgs a{};
-    a.M = s->Hth;
+    a.M = t.Hth;
     a.ld = s->ld;
     a.N = s->N;
     a.R = 1;
-    a.partial = s->partial;
-    a.counters = s->counters;
+    a.partial = t.partial;
+    a.counters = t.counters;
     a.split = matvec_split_for(s->N, 1);
     a.tri = TRI_FULL;
     a.epi = EPI_PLAIN;
-    a.x = s->lv; a.x_stride = 1; a.x_off = 0;
-    a.out = s->lw; a.out_stride = 1; a.out_off = 0;
+    a.x = t.lv; a.x_stride = 1; a.x_off = 0;
+    a.out = t.lw; a.out_stride = 1; a.out_off = 0;
     for (int j = 0; j < m; ++j) {
         launch_matvec(a, st);
-        launch_lanczos_step(s->N, s->lv, s->lvp, s->lw, s->lal, s->lbe, j, st);
+        launch_lanczos_step(s->N, t.lv, t.lvp, t.lw, t.lal, t.lbe, j, st);
     }
     DH_HIP(hipGetLastError());
     double al[LANCZOS_MAX], be[LANCZOS_MAX + 1];
-    DH_HIP(hipMemcpyAsync(al, s->lal, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-    DH_HIP(hipMemcpyAsync(be, s->lbe, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, st));
+    DH_HIP(hipMemcpyAsync(al, t.lal, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    DH_HIP(hipMemcpyAsync(be, t.lbe, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, st));
     DH_HIP(hipStreamSynchronize(st));
-    lanczos_bounds_host(al, be, m, s->rel_margin, s->bounds_h);
+    lanczos_bounds_host(al, be, m, s->rel_margin, t.bounds);
+    return DH_OK;
+}
+
+// the bounds of a finished build become the system's
+static void take_bounds(dh_system* s, const double* b6) {
+    for (int i = 0; i < 6; ++i) s->bounds_h[i] = b6[i];
     s->emin = s->bounds_h[0];
     s->emax = s->bounds_h[1];
     s->widen = 1.0;
     s->coef_dt = -1.0;  // force new coefficients
+}
+
+static int compute_bounds(dh_system* s) {
+    OpsTarget t = main_target(s);
+    int rc = bounds_into(s, t);
+    if (rc) return rc;
+    take_bounds(s, s->bounds_h);
     return DH_OK;
 }
 
@@ -605,6 +677,56 @@ static void launch_hprime(dh_system* s, const double* linv) {
     s->hprime_ready = true;
 }
 
+// S = L L^T, L^-1, W = L^-1 h, H_orth = W L^-T from t.S and t.h, all on t.st
+// (host wait for the Cholesky status); with hprime (in-place builds only) also
+// H' = L^-T W into the context's Hp.  Q1 and Q2 come back mirrored.
+static int factor_into(dh_system* s, const OpsTarget& t, bool hprime) {
+    hipStream_t st = t.st;
+    const int ld = s->ld;
+    launch_copy_matrix(t.S, t.Q1, (long)ld * ld, st);
+    DH_HIP(hipMemsetAsync(t.info, 0, sizeof(int), st));
+    launch_cholesky(t.Q1, ld, t.Dinv, t.info, st);
+    int info = 0;
+    DH_HIP(hipMemcpyAsync(&info, t.info, sizeof(int), hipMemcpyDeviceToHost, st));
+    DH_HIP(hipStreamSynchronize(st));
+    if (info != 0) {
+        dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
+        return DH_ERR_NOT_SPD;
+    }
+    launch_trtri_lower(t.Q1, t.Q2, t.Dinv, ld, t.W, st);  // W is free until W = L^-1 h
+    {   // W = L^-1 h
+        Gemm g{};
+        g.M = ld; g.N = ld; g.K = ld;
+        g.A = t.Q2; g.lda = ld;
+        g.B = t.h; g.ldb = ld;
+        g.C = t.W; g.ldc = ld;
+        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_ROWTILE; g.batch = 1;
+        launch_gemm(g, st);
+    }
+    {   // H_orth = W L^-T (lower tiles), then mirror
+        Gemm g{};
+        g.M = ld; g.N = ld; g.K = ld;
+        g.A = t.W; g.lda = ld;
+        g.B = t.Q2; g.ldb = ld; g.transB = 1;
+        g.C = t.Hth; g.ldc = ld;
+        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_COLTILE; g.lower_only = 1; g.batch = 1;
+        launch_gemm(g, st);
+        launch_symmetrize_from_lower(t.Hth, ld, st);
+    }
+    if (hprime) launch_hprime(s, t.Q2);  // Q2 is not mirrored yet
+    launch_symmetrize_from_lower(t.Q1, ld, st);
+    launch_symmetrize_from_lower(t.Q2, ld, st);
+    DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+// the FP32 copies of t.Hth that the precision modes read
+static void f32_copies_into(dh_system* s, const OpsTarget& t) {
+    const int ld = s->ld;
+    if (s->precision == 3) launch_split_f32_bf16(t.Hth, t.H32, t.Hlo, ld, t.st);
+    else if (s->precision != 0) launch_to_f32(t.Hth, t.H32, (long)ld * ld, t.st);
+}
+
 extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
     CHECK_ARG(s, "dh_orthogonalize: null system");
     dh_context* c = s->ctx;
@@ -613,56 +735,21 @@ extern "C" int dh_orthogonalize(dh_system* s, int want_hprime) {
         return DH_ERR_STATE;
     }
     DH_HIP(hipSetDevice(c->device));
-    hipStream_t st = c->stream;
-    const int ld = s->ld;
-    launch_copy_matrix(c->S, s->Q1, (long)ld * ld, st);
-    DH_HIP(hipMemsetAsync(c->d_info, 0, sizeof(int), st));
-    launch_cholesky(s->Q1, ld, c->Dinv, c->d_info, st);
-    int info = 0;
-    DH_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-    DH_HIP(hipStreamSynchronize(st));
-    if (info != 0) {
+    OpsTarget t = main_target(s);
+    int rc = factor_into(s, t, want_hprime != 0);
+    if (rc) {
         s->ops_ready = false;
-        dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
-        return DH_ERR_NOT_SPD;
+        return rc;
     }
-    launch_trtri_lower(s->Q1, s->Q2, c->Dinv, ld, c->W, st);  // W is free until W = L^-1 h
-    {   // W = L^-1 h
-        Gemm g{};
-        g.M = ld; g.N = ld; g.K = ld;
-        g.A = s->Q2; g.lda = ld;
-        g.B = c->h; g.ldb = ld;
-        g.C = c->W; g.ldc = ld;
-        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_ROWTILE; g.batch = 1;
-        launch_gemm(g, st);
-    }
-    {   // H_orth = W L^-T (lower tiles), then mirror
-        Gemm g{};
-        g.M = ld; g.N = ld; g.K = ld;
-        g.A = c->W; g.lda = ld;
-        g.B = s->Q2; g.ldb = ld; g.transB = 1;
-        g.C = s->Hth; g.ldc = ld;
-        g.alpha = 1.0; g.beta = 0.0; g.kmode = K_BELOW_COLTILE; g.lower_only = 1; g.batch = 1;
-        launch_gemm(g, st);
-        launch_symmetrize_from_lower(s->Hth, ld, st);
-    }
-    s->hprime_ready = false;
+    s->hprime_ready = want_hprime != 0;
     s->w_ready = true;
-    if (want_hprime) launch_hprime(s, s->Q2);  // Q2 is not mirrored yet
-    launch_symmetrize_from_lower(s->Q1, ld, st);
-    launch_symmetrize_from_lower(s->Q2, ld, st);
-    DH_HIP(hipGetLastError());
     if (!s->tuned) {
         int trc = autotune_system(s);
         if (trc) return trc;
     }
-    s->h32_ready = false;
-    if (s->precision != 0) {
-        if (s->precision == 3) launch_split_f32_bf16(s->Hth, s->H32, s->Hlo, ld, st);
-        else launch_to_f32(s->Hth, s->H32, (long)ld * ld, st);
-        s->h32_ready = true;
-    }
-    int rc = compute_bounds(s);
+    f32_copies_into(s, t);
+    s->h32_ready = s->precision != 0;
+    rc = compute_bounds(s);
     if (rc) return rc;
     s->ops_ready = true;
     s->phi_dirty = true;
@@ -946,7 +1033,7 @@ static int chebyshev_series(dh_system* s) {
     hipStream_t st = s->ctx->stream;
     const int variant = matvec_variant_for(s->N, 2 * s->W, TRI_FULL);
     for (auto& g : s->graphs)
-        if (g.exec && g.K == s->K && g.phi == s->phi && g.c == s->coef_c && g.r == s->coef_r &&
+        if (g.exec && g.K == s->K && g.phi == s->phi && g.H == s->Hth && g.c == s->coef_c && g.r == s->coef_r &&
             g.prec == s->precision && g.variant == variant) {
             DH_HIP(hipGraphLaunch(g.exec, st));
             return DH_OK;
@@ -978,6 +1065,7 @@ static int chebyshev_series(dh_system* s) {
     }
     g.K = s->K;
     g.phi = s->phi;
+    g.H = s->Hth;
     g.c = s->coef_c;
     g.r = s->coef_r;
     g.prec = s->precision;
@@ -1107,22 +1195,104 @@ extern "C" int dh_propagate(dh_system* s, double t_init, double t_max, int* n_te
 
 // Operators from the coordinates already in s->coords (device): S and h, then
 // the orthogonalisation (dh_build_operators after dh_set_coords).
-// The stream is idle when this starts (the interval before it ended with a host
-// wait), so the host time to the bounds, which wait for the device, is the
-// rebuild's device time: stats[8] accumulates it.
-static int rebuild_from_device_coords(dh_system* s, double cutoff, double s_drop) {
+// Complex elements of partial sums a matvec of R columns at N may need, over
+// every kernel variant the autotuner can pick for the shape.
+static long matvec_partial_cplx(int N, int R) {
+    const int ld = round_ld(N);
+    long part = (long)matvec_split_for(N, R) * ld * std::min(R, 16);  // split-K column-block variants
+    if (sym_supported(R, TRI_FULL)) part = std::max(part, std::max(sym_partial_cplx(ld, R), sym2_partial_cplx(ld, R)));
+    return std::max(part, (gemm3_partial_doubles(N, ld, R) + 1) / 2);
+}
+
+// Side stream, second workspace and the system's shadow operator set (allocated
+// once, kept for later calls).
+static int ensure_side(dh_system* s) {
+    dh_context* c = s->ctx;
+    if (!c->side) DH_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (c->ws2_ld < c->ws_ld) {
+        free_ws2(c);
+        const size_t n = (size_t)c->ws_ld * c->ws_ld;
+        DALLOC(c->S2, n);
+        DALLOC(c->h2, n);
+        DALLOC(c->W2, n);
+        DALLOC(c->Dinv2, (size_t)c->ws_ld * TILE);
+        DALLOC(c->info2, 1);
+        c->ws2_ld = c->ws_ld;
+    }
+    auto& h = s->sh;
+    const size_t mat = (size_t)s->ld * s->ld;
+    if (!h.allocated) {
+        DALLOC(h.Q1, mat);
+        DALLOC(h.Q2, mat);
+        DALLOC(h.Hth, mat);
+        DALLOC(h.coords, (size_t)s->n_atoms * 3);
+        DALLOC(h.lv, (size_t)s->ld);
+        DALLOC(h.lvp, (size_t)s->ld);
+        DALLOC(h.lw, (size_t)s->ld);
+        DALLOC(h.lal, LANCZOS_MAX + 2);
+        DALLOC(h.lbe, LANCZOS_MAX + 2);
+        DALLOC(h.partial, (size_t)matvec_partial_cplx(s->N, 1));
+        DALLOC(h.counters, (size_t)s->ld);
+        launch_zero_u32(h.counters, s->ld, c->side);
+        launch_zero_cplx(h.lw, s->ld, c->side);
+        h.allocated = true;
+    }
+    if (s->precision != 0 && !h.H32) DALLOC(h.H32, mat);
+    if (s->precision == 3 && !h.Hlo) DALLOC(h.Hlo, mat);
+    return DH_OK;
+}
+
+// The operators of the geometry at d_geo (device, n_atoms x 3) into the shadow
+// set, on the side stream: S and h, Cholesky, L^-1, H_orth, the precision
+// mode's FP32 copies and the Lanczos bounds.  The host waits on the side stream
+// only, so the main stream's queued steps run meanwhile; the side stream is
+// idle when this returns.
+static int side_build(dh_system* s, const double* d_geo, double cutoff, double s_drop) {
     const auto t0 = std::chrono::steady_clock::now();
-    s->stats[7] += 1;
-    int rc = ensure_ws(s->ctx, s->ld);
-    if (rc) return rc;
-    s->ctx->ws_owner = s;
-    if (launch_build_SH(s->bd, s->N, s->ld, s->ctx->S, s->ctx->h, cutoff, s_drop, s->ctx->stream)) {
+    dh_context* c = s->ctx;
+    auto& h = s->sh;
+    DH_HIP(hipMemcpyAsync(h.coords, d_geo, sizeof(double) * 3 * s->n_atoms, hipMemcpyDeviceToDevice, c->side));
+    BasisDev b2 = s->bd;
+    b2.coords = h.coords;
+    if (launch_build_SH(b2, s->N, s->ld, c->S2, c->h2, cutoff, s_drop, c->side)) {
         dh::set_error("dh_propagate_steps: kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         return DH_ERR_HIP;
     }
-    rc = dh_orthogonalize(s, 0);
+    OpsTarget t{c->side, c->S2, c->h2, c->W2, c->Dinv2, c->info2, h.Q1, h.Q2, h.Hth, h.H32, h.Hlo,
+                h.lv, h.lvp, h.lw, h.partial, h.lal, h.lbe, h.counters, h.bounds};
+    int rc = factor_into(s, t, false);
+    if (rc) return rc;
+    f32_copies_into(s, t);
+    rc = bounds_into(s, t);
     s->stats[8] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// The shadow operators become the system's (host pointer swaps; both streams
+// are idle): as after dh_build_operators on the new geometry.
+static void side_commit(dh_system* s) {
+    dh_context* c = s->ctx;
+    auto& h = s->sh;
+    std::swap(s->Q1, h.Q1);
+    std::swap(s->Q2, h.Q2);
+    std::swap(s->Hth, h.Hth);
+    std::swap(s->H32, h.H32);
+    std::swap(s->Hlo, h.Hlo);
+    std::swap(s->coords, h.coords);
+    s->bd.coords = s->coords;
+    std::swap(c->S, c->S2);
+    std::swap(c->h, c->h2);
+    std::swap(c->W, c->W2);
+    std::swap(c->Dinv, c->Dinv2);
+    c->ws_owner = s;
+    take_bounds(s, h.bounds);
+    s->have_coords = true;
+    s->ops_ready = true;
+    s->phi_dirty = true;
+    s->w_ready = true;
+    s->hprime_ready = false;
+    s->h32_ready = s->precision != 0;
+    s->stats[7] += 1;
 }
 
 // Device buffers of one dh_propagate_steps call, freed on every return path.
@@ -1158,20 +1328,19 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
     DALLOC(d.ck_out, (size_t)s->ld * 3 * W);
     const size_t geo_n = (size_t)s->n_atoms * 3;
     if (n_geo > 0) {
+        if ((rc = ensure_side(s))) return rc;
         DALLOC(d.geo, geo_n * n_geo);
         // every geometry goes to the device once; a rebuild reads its slice there
         DH_HIP(hipMemcpyAsync(d.geo, coords, sizeof(double) * geo_n * n_geo, hipMemcpyHostToDevice, st));
+        DH_HIP(hipStreamSynchronize(st));  // the side stream reads them
     }
     std::vector<double> h0((size_t)n_steps * R), h1((size_t)n_steps * R);
     const double drift_tol = fp32_products(s) ? 1e-4 : 1e-9;
     int last_terms = 0;
     for (int j = 0; j < n_steps;) {
-        if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) {
-            DH_HIP(hipMemcpyAsync(s->coords, d.geo + geo_n * (j / rebuild_every - 1), sizeof(double) * geo_n,
-                                  hipMemcpyDeviceToDevice, st));
-            s->have_coords = true;
-            if ((rc = rebuild_from_device_coords(s, cutoff, s_drop))) return rc;
-        }
+        // the operators of this step's geometry were built while the previous
+        // interval's steps ran
+        if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) side_commit(s);
         const int jend = rebuild_every > 0 ? std::min(n_steps, (j / rebuild_every + 1) * rebuild_every) : n_steps;
         if (s->phi_dirty) {
             convert_ao_to_phi(s);
@@ -1203,6 +1372,13 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                                   st));
         }
         DH_HIP(hipGetLastError());
+        // the next geometry's operators, on the side stream, while these steps run
+        if (rebuild_every > 0 && jend < n_steps && jend % rebuild_every == 0) {
+            if ((rc = side_build(s, d.geo + geo_n * (jend / rebuild_every - 1), cutoff, s_drop))) {
+                hipStreamSynchronize(st);
+                return rc;
+            }
+        }
         const size_t cnt = (size_t)(jend - j) * R;
         DH_HIP(hipMemcpyAsync(h0.data() + (size_t)j * R, d.n0 + (size_t)j * R, sizeof(double) * cnt,
                               hipMemcpyDeviceToHost, st));

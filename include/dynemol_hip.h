@@ -138,10 +138,14 @@ int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
  * every wavepacket by dt (ps) as dh_propagate does; with rebuild_every = r > 0
  * the operators are rebuilt (S, h, Cholesky, H_orth, bounds) before steps r,
  * 2r, ... from geometry g = step / r - 1 of coords (n_atoms x 3 per geometry,
- * (n_steps - 1) / r geometries, all copied to the device at the start).  The
- * host waits once per rebuild interval, where the norms of the interval's
+ * (n_steps - 1) / r geometries, all copied to the device at the start).  Each
+ * rebuild runs on a second stream of the context, into a second set of
+ * operator buffers, while the interval before it propagates, and is swapped in
+ * at its step (the system and the context then hold about 7 N^2 doubles more).
+ * The host waits once per rebuild interval, where the norms of the interval's
  * steps are checked; an interval with a drifting step is redone step by step
- * with wider bounds.  pops (n_steps x n_wp x (n_frag + 1)) and erg (n_steps x
+ * with wider bounds.  A rebuild takes the context workspace over, as
+ * dh_build_operators does.  pops (n_steps x n_wp x (n_frag + 1)) and erg (n_steps x
  * n_wp x 2) receive each step's populations and energies (may be NULL); the
  * final state is read as after dh_propagate.  Same kernels and results as
  * n_steps calls of dh_propagate(sys, 0, dt) with dh_set_coords +
@@ -173,8 +177,8 @@ int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, dou
 int dh_ehrenfest_kernel(dh_system *sys, int wp_el, int wp_hl, const double *X, int ldx, double *K, int ldk);
 /* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
  * retries, last norm drift, term-series GPU ms (HIP events, when timing is
- * on), term launches timed, operator rebuilds, ms of the rebuilds inside
- * dh_propagate_steps]. */
+ * on), term launches timed, operator rebuilds, host ms of the rebuilds that
+ * dh_propagate_steps ran beside the propagation]. */
 int dh_get_stats(dh_system *sys, double *out9);
 /* Record HIP events around each step's Chebyshev term series (on the
  * context stream that runs them); no counterpart in the reference, whose
