@@ -29,6 +29,10 @@ using namespace dh;
 static const double H_BAR = 6.58264e-4;  // eV*ps, constants_m.f:23
 static const int KMAX_ALLOC = 4096;      // Chebyshev terms supported (r*dt up to ~3500)
 static const int LANCZOS_MAX = 120;
+// Chebyshev series truncation: the last term kept has |J_k(r dt)| above this.
+// The neglected tail is below 2.2e-16 of the state's norm, under the FP64
+// rounding the ~10^2 terms of a step accumulate.
+static const double BESSEL_TOL = 1e-16;
 
 // Scratch of dh_chebyshev_apply_noalloc: one copy per device, held until the
 // process ends.  12 MiB covers the work layout of N <= 4096 with R <= 4 and of
@@ -109,7 +113,7 @@ struct dh_system {
     double bounds_h[6] = {0, 0, 0, 0, 0, 0};
     double emin = 0, emax = 0;
     int lanczos_steps = 48;
-    double rel_margin = 0.02;
+    double rel_margin = 0.01;  // of the Ritz width on each side, beyond the Ritz residuals
     double widen = 1.0;
     double coef_dt = -1.0, coef_c = 0.0, coef_r = 0.0;
     int K = 0;
@@ -908,7 +912,7 @@ static int ensure_coefficients(dh_system* s, double dt) {
     const double r = 0.5 * (s->emax - s->emin) * s->widen;
     if (dt == s->coef_dt && c == s->coef_c && r == s->coef_r) return DH_OK;
     hipStream_t st = s->ctx->stream;
-    launch_bessel(r * dt, c * dt, KMAX_ALLOC, 1e-18, s->J, s->coef, s->kout, st);
+    launch_bessel(r * dt, c * dt, KMAX_ALLOC, BESSEL_TOL, s->J, s->coef, s->kout, st);
     int k = 0;
     DH_HIP(hipGetLastError());
     DH_HIP(hipMemcpyAsync(&k, s->kout, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1577,7 +1581,7 @@ static int chebyshev_apply_impl(dh_context* ctx, int N, const double* H, int ldh
     for (int attempt = 0; attempt < 4; ++attempt) {
         const double c = 0.5 * (bounds[1] + bounds[0]);
         const double r = 0.5 * (bounds[1] - bounds[0]) * widen;
-        launch_bessel(r * dt, c * dt, KMAX_ALLOC, 1e-18, w.J, w.coef, w.kout, st);
+        launch_bessel(r * dt, c * dt, KMAX_ALLOC, BESSEL_TOL, w.J, w.coef, w.kout, st);
         int K = 0;
         DH_HIP(hipGetLastError());
         DH_HIP(hipMemcpyAsync(&K, w.kout, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -86,7 +86,7 @@ int dh_orthogonalize(dh_system *sys, int want_hprime);
 int dh_get_matrix(dh_system *sys, int which, double *out, int ld);
 /* [E_min, E_max, theta_min, theta_max, residual_min, residual_max] (eV). */
 int dh_get_bounds(dh_system *sys, double *out6);
-/* Override the Lanczos step count and relative safety margin (defaults 48, 0.02). */
+/* Override the Lanczos step count and relative safety margin (defaults 48, 0.01). */
 int dh_set_lanczos(dh_system *sys, int steps, double rel_margin);
 
 /* Arithmetic of the Chebyshev term products: 0 = FP64 (default); 1 = mixed:
