@@ -415,17 +415,17 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
     }
     hipStream_t st = ctx->stream;
     if (n_atoms > 0) {
-        DH_HIP(hipMemcpy(d_at_ns, at_ns.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_at_s0, at_s0.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_n, sh_n.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_l, sh_l.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_nz, sh_nz.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_o, sh_orb0.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_z, sh_z.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_sh_c, sh_c.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_ip, orb_ip, sizeof(double) * N, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_kwh, orb_kwh, sizeof(double) * N, hipMemcpyHostToDevice));
-        DH_HIP(hipMemcpy(d_vs, orb_vshift, sizeof(double) * N, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpyAsync(d_at_ns, at_ns.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_at_s0, at_s0.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_n, sh_n.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_l, sh_l.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_nz, sh_nz.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_o, sh_orb0.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_z, sh_z.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_sh_c, sh_c.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_ip, orb_ip, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_kwh, orb_kwh, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        DH_HIP(hipMemcpyAsync(d_vs, orb_vshift, sizeof(double) * N, hipMemcpyHostToDevice, st));
         // atoms ordered by basis signature (shells' n, l, zetas): the S build maps
         // a wavefront onto pairs of one element pair, so its lanes take the same branches
         std::vector<int> order(n_atoms);
@@ -439,7 +439,8 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
             return k;
         };
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
-        DH_HIP(hipMemcpy(d_order, order.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice, st));
+        DH_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
         SALLOC(s->coords, (size_t)n_atoms * 3);
     }
     s->bd = BasisDev{n_atoms, s->coords, d_at_ns, d_at_s0, d_sh_n, d_sh_l, d_sh_nz, d_sh_o, d_sh_z, d_sh_c,
@@ -449,7 +450,8 @@ extern "C" int dh_system_create(dh_context* ctx, int n_atoms, const int* first_o
         std::vector<int> fr(N, 0);
         if (orb_frag)
             for (int i = 0; i < N; ++i) fr[i] = orb_frag[i];
-        DH_HIP(hipMemcpy(s->frag, fr.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        DH_HIP(hipMemcpyAsync(s->frag, fr.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
+        DH_HIP(hipStreamSynchronize(st));
     }
     const size_t mat = (size_t)ld * ld;
     SALLOC(s->Q1, mat);
@@ -2102,7 +2104,8 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
         BALLOC(b->d_t2_desc, (size_t)t2);
         BALLOC(b->d_t2_idx, (size_t)t2);
         BALLOC(b->d_sysS, (size_t)n);
-        if (hipMemcpy(b->d_sysS, sS.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpyAsync(b->d_sysS, sS.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
             dh_batch_destroy(b);
             dh::set_error("dh_batch_create: copy of the segment table failed");
             return DH_ERR_HIP;

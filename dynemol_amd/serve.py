@@ -95,6 +95,9 @@ class _Lane:
     def _take(self, req):
         w = padded_wp(len(req.psi_ket))
         s = hip.System(self.ctx, req.basis, n_wp=w, n_frag=req.n_frag)
+        # lanes run concurrently: no stream capture, which must not overlap the
+        # other lanes' allocations and frees
+        s.set_graph(False)
         if req.mixed:
             s.set_precision(hip.DH_PREC_MIXED_CORRECTED)
         s.build(req.coords)
@@ -123,6 +126,7 @@ class _Lane:
                 if old is not None:
                     old.close()
                 self._batches[key] = hip.Batch(self.ctx, [r.system for r in reqs])
+                self._batches[key].set_graph(False)
                 self._batch_keys[key] = ids
             self._batches[key].propagate(0.0, key[1])
             for r in reqs:

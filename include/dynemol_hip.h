@@ -9,6 +9,10 @@
  *
  * Threading: a context owns one HIP stream; calls on one context must not
  * overlap.  Systems created on a context run on that context's stream.
+ * Several contexts may be driven from several host threads at once; the
+ * term series' stream capture (dh_set_graph, dh_batch_set_graph; on by
+ * default for N <= 2048) must then be off, since a capture must not overlap
+ * another thread's device allocations and frees.
  */
 #ifndef DYNEMOL_HIP_H
 #define DYNEMOL_HIP_H
