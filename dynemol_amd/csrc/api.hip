@@ -33,6 +33,9 @@ static const int LANCZOS_MAX = 120;
 // The neglected tail is below 2.2e-16 of the state's norm, under the FP64
 // rounding the ~10^2 terms of a step accumulate.
 static const double BESSEL_TOL = 1e-16;
+// K slices of the mixed modes' library GEMMs (ld is a multiple of 64, so the
+// slices are equal and whole)
+static const int LIB_SLICES = 4;
 
 // Scratch of dh_chebyshev_apply_noalloc: one copy per device, held until the
 // process ends.  12 MiB covers the work layout of N <= 4096 with R <= 4 and of
@@ -142,7 +145,9 @@ struct dh_system {
     // each T_k as its epilogue writes it), alternating between two buffers
     float* x32[2] = {nullptr, nullptr};
     bool h32_ready = false;
-    unsigned short* Hlo = nullptr;  // mode 3: bf16 residual H_orth - H32 (the GEMM's 16-k block layout)
+    unsigned short* Hlo = nullptr;  // mode 3: bf16 residual H_orth - H32, column-major
+    // mode 3: bf16 copies of the term inputs (the residual's products), alternating like x32
+    unsigned short* xb16[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> evpool;  // per-step series events of dh_propagate_steps (timing on)
     // the next geometry's operators, built on the context's side stream while
@@ -255,6 +260,8 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->Hlo) hipFree(s->Hlo);
     for (float* p : s->x32)
         if (p) hipFree(p);
+    for (unsigned short* p : s->xb16)
+        if (p) hipFree(p);
     for (auto& g : s->graphs)
         if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
@@ -297,6 +304,8 @@ static long partial_cplx_for(int N, int n_wp) {
         if (sym_supported(R, TRI_FULL)) n = std::max(n, std::max(sym_partial_cplx(ld, R), sym2_partial_cplx(ld, R)));
         n = std::max(n, (gemm3_partial_doubles(N, ld, R) + 1) / 2);
     }
+    // the mixed modes' library GEMM slices: FP32 and residual, LIB_SLICES x 2R x ld floats each
+    n = std::max(n, (long)LIB_SLICES * 2 * (2 * n_wp) * ld * 2 / 4);
     return n;
 }
 
@@ -316,6 +325,7 @@ extern "C" long long dh_system_bytes_prec(int n_orb, int n_wp, int precision) {
     if (precision != 0) b += ld * ld * 4;                       // H32
     if (precision == 3) b += ld * ld * 2;                       // bf16 residual
     if (precision == 1 || precision == 3) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
+    if (precision == 3) b += 2LL * ld * 4 * n_wp * 2;                     // xb16[2]
     return b;
 }
 
@@ -729,7 +739,7 @@ static int factor_into(dh_system* s, const OpsTarget& t, bool hprime) {
 // the FP32 copies of t.Hth that the precision modes read
 static void f32_copies_into(dh_system* s, const OpsTarget& t) {
     const int ld = s->ld;
-    if (s->precision == 3) launch_split_f32_bf16(t.Hth, t.H32, t.Hlo, ld, t.st);
+    if (s->precision == 3) launch_split_f32_bf16(t.Hth, t.H32, t.Hlo, (long)ld * ld, t.st);
     else if (s->precision != 0) launch_to_f32(t.Hth, t.H32, (long)ld * ld, t.st);
 }
 
@@ -935,19 +945,22 @@ static int ensure_coefficients(dh_system* s, double dt) {
 // the recurrence alternates between x0 and x1.
 struct SeriesOps {
     const double* H;
-    const float* H32;      // precision modes 1, 2 and 3
-    const void* Mlo;       // precision mode 3
+    const float* H32;          // precision modes 1, 2 and 3
+    const unsigned short* Hlo;  // precision mode 3
     int ld, N, R;
     const cplx* phi;
     cplx *x0, *x1, *acc, *partial;
     unsigned* counters;
     const cplx* coef;
-    float* x32[2];         // precision mode 1
+    float* x32[2];              // precision modes 1 and 3
+    unsigned short* xb16[2];    // precision mode 3
+    LibHandle* lib;             // precision modes 1 and 3
     int prec, K;
     double c, r;
 };
 
-static void series_core(const SeriesOps& o, hipStream_t st) {
+
+static int series_core(const SeriesOps& o, hipStream_t st) {
     const int R = o.R;
     const double c = o.c, r = o.r;
     MatvecArgs a{};
@@ -964,20 +977,33 @@ static void series_core(const SeriesOps& o, hipStream_t st) {
     a.acc = o.acc;
     a.coef = o.coef;
     a.M32 = o.H32;
-    a.Mlo = o.Mlo;
     const int prec = o.prec;
-    // mode 1: the FP32 GEMM reads FP32 copies of its input; the step's state
-    // is converted once, then each term's epilogue writes the next one's
-    const bool f32x = prec == 1 || prec == 3;  // FP32 products on FP32 copies of x
-    if (f32x) launch_to_f32(reinterpret_cast<const double*>(o.phi), o.x32[0], (long)o.ld * 2 * R, st);
-    auto term = [&](MatvecArgs args) {
+    // modes 1 and 3: the FP32 products read FP32 copies of their input (and mode
+    // 3's residual products bf16 copies); the step's state is converted once,
+    // then each term's epilogue writes the next one's
+    const bool f32x = prec == 1 || prec == 3;
+    const long nx = (long)o.ld * 2 * R;
+    if (f32x) launch_to_f32(reinterpret_cast<const double*>(o.phi), o.x32[0], nx, st);
+    if (prec == 3) launch_to_bf16(o.x32[0], o.xb16[0], nx, st);
+    float* Phi = reinterpret_cast<float*>(o.partial);
+    float* Plo = Phi + (long)LIB_SLICES * 2 * R * o.ld;
+    auto term = [&](MatvecArgs args) -> int {
         if (f32x) {
-            args.x32 = o.x32[(args.k - 1) & 1];
+            const int in = (args.k - 1) & 1;
+            args.x32 = o.x32[in];
             args.out32 = o.x32[args.k & 1];
-        }
-        if (f32x) launch_gemm3(args, prec, reinterpret_cast<double*>(o.partial), st);
-        else if (prec == 2) launch_term_f32h(args, st);
+            if (prec == 3) args.outb16 = o.xb16[args.k & 1];
+            const int KS = o.ld / LIB_SLICES;
+            int rc = lib_gemm_slices_f32(o.lib, o.ld, 2 * R, KS, LIB_SLICES, o.H32, o.ld, o.x32[in], 2 * R, Phi,
+                                         o.ld);
+            if (!rc && prec == 3)
+                rc = lib_gemm_slices_bf16(o.lib, o.ld, 2 * R, KS, LIB_SLICES, o.Hlo, o.ld, o.xb16[in], 2 * R, Plo,
+                                          o.ld);
+            launch_lib_reduce(args, Phi, prec == 3 ? Plo : nullptr, LIB_SLICES, st);
+            return rc;
+        } else if (prec == 2) launch_term_f32h(args, st);
         else launch_matvec(args, st);
+        return DH_OK;
     };
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
     a.epi = EPI_CHEB_FIRST;
@@ -986,7 +1012,7 @@ static void series_core(const SeriesOps& o, hipStream_t st) {
     a.alpha = 1.0 / r;
     a.beta = -c / r;
     a.k = 1;
-    term(a);
+    if (int rc = term(a)) return rc;
     const cplx* prev2 = o.phi;
     cplx* prev = o.x1;
     for (int k = 2; k <= o.K; ++k) {
@@ -998,10 +1024,11 @@ static void series_core(const SeriesOps& o, hipStream_t st) {
         a.alpha = 2.0 / r;
         a.beta = -2.0 * c / r;
         a.k = k;
-        term(a);
+        if (int rc = term(a)) return rc;
         prev2 = prev;
         prev = outb;
     }
+    return DH_OK;
 }
 
 // One Chebyshev evaluation phi <- sum_k a_k T_k(Hh) phi, result in s->acc.
@@ -1009,7 +1036,7 @@ static int series_launches(dh_system* s) {
     SeriesOps o{};
     o.H = s->Hth;
     o.H32 = s->H32;
-    o.Mlo = s->Hlo;
+    o.Hlo = s->Hlo;
     o.ld = s->ld;
     o.N = s->N;
     o.R = 2 * s->W;
@@ -1022,11 +1049,14 @@ static int series_launches(dh_system* s) {
     o.coef = s->coef;
     o.x32[0] = s->x32[0];
     o.x32[1] = s->x32[1];
+    o.xb16[0] = s->xb16[0];
+    o.xb16[1] = s->xb16[1];
+    o.lib = s->ctx->lib;
     o.prec = s->precision;
     o.K = s->K;
     o.c = s->coef_c;
     o.r = s->coef_r;
-    series_core(o, s->ctx->stream);
+    if (int rc = series_core(o, s->ctx->stream)) return rc;
     DH_HIP(hipGetLastError());
     return DH_OK;
 }
@@ -1035,7 +1065,8 @@ static int series_launches(dh_system* s) {
 // assignment, precision, term kernel) and replayed on later steps.  The series
 // is kernels only (no memset nodes, no host synchronisation).
 static int chebyshev_series(dh_system* s) {
-    if (!s->use_graph) return series_launches(s);
+    // the FP32-product modes call rocBLAS, which stays out of stream capture
+    if (!s->use_graph || fp32_products(s)) return series_launches(s);
     hipStream_t st = s->ctx->stream;
     const int variant = matvec_variant_for(s->N, 2 * s->W, TRI_FULL);
     for (auto& g : s->graphs)
@@ -1595,7 +1626,7 @@ static int chebyshev_apply_impl(dh_context* ctx, int N, const double* H, int ldh
         o.K = K;
         o.c = c;
         o.r = r;
-        series_core(o, st);
+        if (int rc = series_core(o, st)) return rc;
         launch_column_norms(N, R, w.acc, w.norms1, st);
         DH_HIP(hipGetLastError());
         DH_HIP(hipMemcpyAsync(n0, w.norms0, sizeof(double) * R, hipMemcpyDeviceToHost, st));
@@ -1895,13 +1926,29 @@ extern "C" int dh_set_precision(dh_system* s, int mode) {
     DH_HIP(hipSetDevice(s->ctx->device));
     if (mode != 0 && !s->H32) DALLOC(s->H32, (size_t)s->ld * s->ld);
     if (mode == 3 && !s->Hlo) DALLOC(s->Hlo, (size_t)s->ld * s->ld);
+    // the FP32 and bf16 term inputs: rows N..ld-1 stay zero (the epilogues write
+    // rows < N only, and the library GEMMs run over all ld rows)
     if (mode == 1 || mode == 3)
         for (float*& p : s->x32)
-            if (!p) DALLOC(p, (size_t)s->ld * 4 * s->W);
+            if (!p) {
+                DALLOC(p, (size_t)s->ld * 4 * s->W);
+                launch_zero_cplx(reinterpret_cast<cplx*>(p), (long)s->ld * s->W, s->ctx->stream);
+            }
+    if (mode == 3)
+        for (unsigned short*& p : s->xb16)
+            if (!p) {
+                DALLOC(p, (size_t)s->ld * 4 * s->W);
+                launch_zero_cplx(reinterpret_cast<cplx*>(p), (long)s->ld * s->W / 2, s->ctx->stream);
+            }
+    // the FP32 products of modes 1 and 3 run on rocBLAS through the context's handle
+    if ((mode == 1 || mode == 3) && !s->ctx->lib && !(s->ctx->lib = lib_create(s->ctx->stream))) {
+        dh::set_error("dh_set_precision: rocBLAS handle creation failed");
+        return DH_ERR_HIP;
+    }
     if (mode != s->precision) s->h32_ready = false;  // mode 3 also needs the residual
     s->precision = mode;
     if (mode != 0 && s->ops_ready && !s->h32_ready) {
-        if (mode == 3) launch_split_f32_bf16(s->Hth, s->H32, s->Hlo, s->ld, s->ctx->stream);
+        if (mode == 3) launch_split_f32_bf16(s->Hth, s->H32, s->Hlo, (long)s->ld * s->ld, s->ctx->stream);
         else launch_to_f32(s->Hth, s->H32, (long)s->ld * s->ld, s->ctx->stream);
         DH_HIP(hipStreamSynchronize(s->ctx->stream));
         s->h32_ready = true;
@@ -1973,8 +2020,8 @@ static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k
                               st);
 }
 
-// Chebyshev terms of a batch: grouped members with K_s >= k are the first
-// `active` of `order`; members in other precision modes run their own series
+// Chebyshev terms of a batch's grouped members: those with K_s >= k are the
+// first `active` of `order`
 static void batch_series_launches(dh_batch* b, hipStream_t st) {
     const int n = b->n;
     int active = n;
@@ -1983,26 +2030,28 @@ static void batch_series_launches(dh_batch* b, hipStream_t st) {
         const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
         batch_term_launch(b, b->sym, kind, active, k, st);
     }
-    for (int i = 0; i < n; ++i)
-        if (b->solo[i]) series_launches(b->sys[i]);
 }
 
 static std::vector<double> batch_graph_key(const dh_batch* b) {
     std::vector<double> key(b->Ks.begin(), b->Ks.end());
     key.push_back(b->sym ? 1.0 : 0.0);
-    for (int i = 0; i < b->n; ++i)
-        if (b->solo[i]) {
-            const dh_system* s = b->sys[i];
-            key.insert(key.end(), {(double)i, s->coef_c, s->coef_r, (double)s->precision,
-                                   (double)reinterpret_cast<uintptr_t>(s->phi)});
-        }
+    for (int i = 0; i < b->n; ++i) key.push_back(b->solo[i] ? 1.0 : 0.0);
     return key;
 }
 
+// members in other precision modes run their own series after the grouped
+// terms, outside the capture (the FP32-product modes call rocBLAS)
+static int batch_solo_series(dh_batch* b) {
+    for (int i = 0; i < b->n; ++i)
+        if (b->solo[i])
+            if (int rc = series_launches(b->sys[i])) return rc;
+    return DH_OK;
+}
+
 static int batch_series(dh_batch* b, hipStream_t st) {
-    if (!b->use_graph) {
+    if (!b->use_graph || b->Kmax == 0) {
         batch_series_launches(b, st);
-        return DH_OK;
+        return batch_solo_series(b);
     }
     std::vector<double> key = batch_graph_key(b);
     if (!b->gexec || b->gkey != key) {
@@ -2029,7 +2078,7 @@ static int batch_series(dh_batch* b, hipStream_t st) {
         b->gkey = key;
     }
     DH_HIP(hipGraphLaunch(b->gexec, st));
-    return DH_OK;
+    return batch_solo_series(b);
 }
 
 extern "C" int dh_batch_set_graph(dh_batch* b, int on) {

@@ -702,6 +702,14 @@ __global__ __launch_bounds__(256) void cheb_gemm2_kernel(MatvecArgs a) {
     }
 }
 
+// bf16 (round to nearest even) of an FP32 value, as the bits a matrix core reads
+__host__ __device__ inline unsigned short bf16_rne(float f) {
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 // Chebyshev epilogue for output row i, column r with y = (H x)_ir:
 //   EPI_PLAIN:       out = y
 //   EPI_CHEB_FIRST:  T = alpha y + beta x; acc = a_0 x + a_1 T; out = T
@@ -731,6 +739,10 @@ __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r,
     if (a.out32) {  // the next term's FP32 operand: the value its GEMM would round on load
         a.out32[2 * oi] = (float)T.re;
         a.out32[2 * oi + 1] = (float)T.im;
+    }
+    if (a.outb16) {  // and its bf16 operand for the residual's products (precision mode 3)
+        a.outb16[2 * oi] = bf16_rne((float)T.re);
+        a.outb16[2 * oi + 1] = bf16_rne((float)T.im);
     }
 }
 
@@ -785,13 +797,8 @@ constexpr int G3_T = 128;
 // k-stage depth: 16 (FP64) / 32 (FP32) keeps the double-buffered tiles at 74 KB of
 // LDS, so two workgroups share a CU and hide each other's barrier and load waits
 // PREC: 0 = FP64, 1 = FP32 products and slice sums (mixed), 2 = FP32-stored H
-// widened to FP64 on load, FP64 products (the arithmetic of PREC 0), 3 = PREC 1
-// plus the correction by the FP64 residual of the FP32 operator:
-// H = H32 + H_lo, H_lo = H - fl32(H) stored in bf16 (a.Mlo), H_lo X as bf16
-// matrix-core products (16x16x16, 16 times the FP32 rate) in accumulators of
-// their own, added to the FP32 products in FP64 per slice.  The operator the
-// terms apply is then H to 2^-32 relative instead of 2^-24.
-template <int PREC> struct G3 { static constexpr int BK = (PREC == 1 || PREC == 3) ? 32 : 16; };
+// widened to FP64 on load, FP64 products (the arithmetic of PREC 0)
+template <int PREC> struct G3 { static constexpr int BK = PREC == 1 ? 32 : 16; };
 // row pad: successive k-rows of an operand read start 32 banks apart (FP64) or
 // 16 banks apart (FP32), so the four rows a wave reads do not collide
 constexpr int G3_PAD = 16;
@@ -806,21 +813,9 @@ int gemm3_splits(int N, int R) {
 
 long gemm3_partial_doubles(int N, int ld, int R) { return (long)gemm3_splits(N, R) * ld * 2 * R; }
 
-// bf16 (round to nearest even) of an FP32 value, as the bits the matrix core reads
-__device__ inline unsigned short bf16_rne(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-typedef short v4s __attribute__((ext_vector_type(4)));
-
-// two workgroups per CU (the LDS of two fits; PREC 3's residual accumulators
-// must then fit beside the FP32 ones in the register file)
 template <int PREC>
-__global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __restrict__ P, int KS) {
-    constexpr bool F32 = PREC == 1 || PREC == 3;
-    constexpr bool LO = PREC == 3;
+__global__ __launch_bounds__(256) void gemm3_kernel(MatvecArgs a, double* __restrict__ P, int KS) {
+    constexpr bool F32 = PREC == 1;
     using T = typename std::conditional<F32, float, double>::type;
     constexpr int G3_BK = G3<PREC>::BK;
     __shared__ T As[2][G3_BK][G3_T + G3_PAD];
@@ -841,38 +836,6 @@ __global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __r
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[p][q][v] = (T)0;
-    // PREC 3: the residual's products.  H_lo is stored in 16-k blocks as
-    // Mlo[((kb * 4 + g) * ld + i) * 4 + j] = H_lo[i][16 kb + 4 j + g], so lane l
-    // (g = l >> 4) loads its four k values of orbital i as one 8-byte word; the
-    // matching x values are the FP32 operands bv its FP32 products read at
-    // ks = 16 kb + 4 j (row ks + g), rounded to bf16.  Both operands agree on
-    // which physical k each of the 16 products of an MFMA takes.
-    v4f acl[4][4];
-    v4s hlo[2][4];
-    v4s xb[4];
-    if constexpr (LO) {
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acl[p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-    const unsigned short* Mlo = reinterpret_cast<const unsigned short*>(a.Mlo);
-    auto lo_load = [&](int k0, v4s (&dst)[2][4]) {
-        if constexpr (LO) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int kb = k0 / 16 + h;
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    const int gi = i0 + wi + p * 16 + (lane & 15);
-                    v4s v = {0, 0, 0, 0};
-                    if (gi < a.ld && 16 * kb < a.ld)
-                        v = *reinterpret_cast<const v4s*>(Mlo + (((long)kb * 4 + (lane >> 4)) * a.ld + gi) * 4);
-                    dst[h][p] = v;
-                }
-            }
-        }
-    };
     constexpr int LPT = G3_BK * G3_T / 256;  // elements of each tile per thread per stage
     T ra[LPT], rb[LPT];
     // H tile: rows i0..i0+127 (lane-contiguous), k0..k0+BK-1; X tile: k0..k0+BK-1, columns c0..c0+127
@@ -913,8 +876,6 @@ __global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __r
     __syncthreads();
     for (int k0 = klo; k0 < khi; k0 += G3_BK) {
         const bool more = k0 + G3_BK < khi;
-        // the stage's residual words: first read after four k-steps of FP32 products
-        lo_load(k0, hlo);
         if (more) gload(k0 + G3_BK);
 #pragma unroll
         for (int ks = 0; ks < G3_BK; ks += 4) {
@@ -932,19 +893,6 @@ __global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __r
                     else
                         acc[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[q], av[p], acc[p][q], 0, 0, 0);
                 }
-            if constexpr (LO) {
-                const int j = (ks >> 2) & 3;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) xb[q][j] = (short)bf16_rne(bv[q]);
-                if (j == 3) {
-                    const int h = ks >> 4;
-#pragma unroll
-                    for (int p = 0; p < 4; ++p)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            acl[p][q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xb[q], hlo[h][p], acl[p][q], 0, 0, 0);
-                }
-            }
         }
         if (more) swrite(s ^ 1);
         __syncthreads();
@@ -952,8 +900,7 @@ __global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __r
     }
     // D^T layout: f64 lane l, value v -> column (l >> 4) + 4v, orbital l & 15;
     //             f32 lane l, value v -> column 4 (l >> 4) + v, orbital l & 15.
-    // PREC 1 stores its FP32 slice sums as FP32 (exact, half the bytes); PREC 3
-    // adds the residual's sums to them in FP64.
+    // PREC 1 stores its FP32 slice sums as FP32 (exact, half the bytes).
     using part_t = typename std::conditional<PREC == 1, float, double>::type;
     part_t* Pz = reinterpret_cast<part_t*>(P) + (long)z * C * a.ld;
 #pragma unroll
@@ -964,9 +911,7 @@ __global__ __launch_bounds__(256, 2) void gemm3_kernel(MatvecArgs a, double* __r
             for (int v = 0; v < 4; ++v) {
                 const int gi = i0 + wi + p * 16 + (lane & 15);
                 const int gc = c0 + wc + q * 16 + (!F32 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v);
-                part_t val = (part_t)acc[p][q][v];
-                if constexpr (LO) val += (double)acl[p][q][v];
-                if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = val;
+                if (gi < N && gc < C) Pz[(long)gc * a.ld + gi] = (part_t)acc[p][q][v];
             }
 }
 
@@ -987,15 +932,48 @@ __global__ __launch_bounds__(256) void gemm3_reduce_kernel(MatvecArgs a, const P
     term_epilogue(a, i, r, y);
 }
 
+// Slices of the library GEMMs of the mixed modes, added in FP64 in slice order:
+// the FP32 products Phi[z] and (mode 3) the residual's products Plo[z], both
+// laid out as gemm3's slices ([z][c][i], ld rows)
+__global__ __launch_bounds__(256) void lib_reduce_kernel(MatvecArgs a, const float* __restrict__ Phi,
+                                                          const float* __restrict__ Plo, int S) {
+    const long n = (long)a.N * a.R;
+    const long e = blockIdx.x * 256L + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e / a.N), i = (int)(e % a.N);
+    const long C = 2L * a.R;
+    cplx y{0.0, 0.0};
+    for (int z = 0; z < S; ++z) {
+        const long o = (long)z * C * a.ld;
+        y.re += (double)Phi[o + (2L * r) * a.ld + i];
+        y.im += (double)Phi[o + (2L * r + 1) * a.ld + i];
+    }
+    if (Plo) {
+        cplx lo{0.0, 0.0};
+        for (int z = 0; z < S; ++z) {
+            const long o = (long)z * C * a.ld;
+            lo.re += (double)Plo[o + (2L * r) * a.ld + i];
+            lo.im += (double)Plo[o + (2L * r + 1) * a.ld + i];
+        }
+        y.re += lo.re;
+        y.im += lo.im;
+    }
+    term_epilogue(a, i, r, y);
+}
+
+void launch_lib_reduce(const MatvecArgs& a, const float* Phi, const float* Plo, int S, hipStream_t st) {
+    const long n = (long)a.N * a.R;
+    hipLaunchKernelGGL(lib_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, Phi, Plo, S);
+}
+
 void launch_gemm3(const MatvecArgs& a, int prec, double* P, hipStream_t st) {
     const int S = gemm3_splits(a.N, a.R);
-    const int BK = (prec == 1 || prec == 3) ? G3<1>::BK : G3<0>::BK;
+    const int BK = prec == 1 ? G3<1>::BK : G3<0>::BK;
     const int KS = (((a.N + S - 1) / S) + BK - 1) / BK * BK;
     const int Sr = (a.N + KS - 1) / KS;
     dim3 grid((a.N + G3_T - 1) / G3_T, (2 * a.R + G3_T - 1) / G3_T, Sr);
     if (prec == 0) hipLaunchKernelGGL((gemm3_kernel<0>), grid, dim3(256), 0, st, a, P, KS);
     else if (prec == 1) hipLaunchKernelGGL((gemm3_kernel<1>), grid, dim3(256), 0, st, a, P, KS);
-    else if (prec == 3) hipLaunchKernelGGL((gemm3_kernel<3>), grid, dim3(256), 0, st, a, P, KS);
     else hipLaunchKernelGGL((gemm3_kernel<2>), grid, dim3(256), 0, st, a, P, KS);
     const long n = (long)a.N * a.R;
     const dim3 rg((unsigned)((n + 255) / 256));
@@ -2416,20 +2394,20 @@ __global__ void renorm_columns_kernel(cplx* x, int N, int R, const double* n0, c
     x[e].im *= f;
 }
 
-// H32 = fl32(H) and the residual H - H32 in bf16, in the 16-k block layout the
-// PREC 3 GEMM reads: Mlo[((kb * 4 + g) * ld + i) * 4 + j] = lo(H[i][16 kb + 4 j + g])
-// (H column-major: H[i][k] at i + ld k).
+// H32 = fl32(H) and the residual H - H32 in bf16, both column-major like H
 __global__ void split_f32_bf16_kernel(const double* __restrict__ H, float* __restrict__ H32,
-                                      unsigned short* __restrict__ Mlo, int ld) {
-    const long n = (long)ld * ld;
+                                      unsigned short* __restrict__ Hlo, long n) {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % ld), k = (int)(e / ld);
         const double h = H[e];
         const float hi = (float)h;
         H32[e] = hi;
-        const int kb = k >> 4, r = k & 15, j = r >> 2, g = r & 3;
-        Mlo[(((long)kb * 4 + g) * ld + i) * 4 + j] = bf16_rne((float)(h - (double)hi));
+        Hlo[e] = bf16_rne((float)(h - (double)hi));
     }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ a, unsigned short* __restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = bf16_rne(a[i]);
 }
 
 __global__ void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, long n) {
@@ -2451,9 +2429,11 @@ void launch_renorm_columns(cplx* x, int N, int R, const double* n0, const double
     const long n = (long)N * R;
     hipLaunchKernelGGL(renorm_columns_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, N, R, n0, n1);
 }
-void launch_split_f32_bf16(const double* H, float* H32, void* Mlo, int ld, hipStream_t st) {
-    hipLaunchKernelGGL(split_f32_bf16_kernel, dim3(4096), dim3(256), 0, st, H, H32,
-                       static_cast<unsigned short*>(Mlo), ld);
+void launch_split_f32_bf16(const double* H, float* H32, unsigned short* Hlo, long n, hipStream_t st) {
+    hipLaunchKernelGGL(split_f32_bf16_kernel, dim3(4096), dim3(256), 0, st, H, H32, Hlo, n);
+}
+void launch_to_bf16(const float* a, unsigned short* b, long n, hipStream_t st) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, st, a, b, n);
 }
 void launch_to_f32(const double* a, float* b, long n, hipStream_t st) {
     hipLaunchKernelGGL(to_f32_kernel, dim3(4096), dim3(256), 0, st, a, b, n);

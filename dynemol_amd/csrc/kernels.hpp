@@ -95,6 +95,14 @@ void lib_destroy(LibHandle* es);
 // C = alpha A B + beta C, column-major, no transposes
 int lib_dgemm_nn(LibHandle* es, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
                  double beta, double* C, int ldc);
+// The mixed-mode term products in S K-slices of KS: Y[z] = A[:, z KS : (z+1) KS] X[z KS : (z+1) KS, :]
+// with A m x (S KS) column-major (FP32, or bf16 for the _bf16 form), X row-major with
+// ldx values per row (the term's state, rows = orbitals), Y[z] column-major m x c
+// (ldy) at stride c * ldy, FP32 products and accumulation within a slice.
+int lib_gemm_slices_f32(LibHandle* es, int m, int c, int KS, int S, const float* A, int lda, const float* X, int ldx,
+                        float* Y, int ldy);
+int lib_gemm_slices_bf16(LibHandle* es, int m, int c, int KS, int S, const unsigned short* A, int lda,
+                         const unsigned short* X, int ldx, float* Y, int ldy);
 // Eigenpairs il..iu (1-based, ascending) of h C = S C E from H_orth and L^-1
 // (Q2, lower triangle): E (device, 2 N doubles; eigenvalue j at E[j - 1]) and
 // C = L^-T V (device, ld x nev); A and Z are ld x ld scratch.
@@ -135,7 +143,7 @@ struct MatvecArgs {
     const float* M32;       // FP32 copy of M (mixed and FP32-storage precision modes; may be null)
     const float* x32;       // mixed mode: FP32 copy of x (same layout), read by the FP32 GEMM; may be null
     float* out32;           // mixed mode: the epilogue also writes T_k here in FP32 (same layout); may be null
-    const void* Mlo;        // precision mode 3: bf16 residual M - fl32(M) in the GEMM's 16-k block layout
+    unsigned short* outb16; // precision mode 3: the epilogue also writes T_k here in bf16 (same layout); may be null
 };
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
@@ -188,8 +196,11 @@ void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* o
 void launch_column_norms(int N, int R, const cplx* x, double* out, hipStream_t st);
 void launch_renorm_columns(cplx* x, int N, int R, const double* n0, const double* n1, hipStream_t st);
 void launch_to_f32(const double* a, float* b, long n, hipStream_t st);
-// H32 = fl32(H) and the bf16 residual H - H32 in the PREC 3 GEMM's layout (2 ld^2 bytes)
-void launch_split_f32_bf16(const double* H, float* H32, void* Mlo, int ld, hipStream_t st);
+// H32 = fl32(H) and the bf16 residual H - H32, both column-major (n elements)
+void launch_split_f32_bf16(const double* H, float* H32, unsigned short* Hlo, long n, hipStream_t st);
+void launch_to_bf16(const float* a, unsigned short* b, long n, hipStream_t st);
+// the mixed modes' library-GEMM slices added in FP64, then the term epilogue
+void launch_lib_reduce(const MatvecArgs& a, const float* Phi, const float* Plo, int S, hipStream_t st);
 void launch_identity_pad(double* S, double* h, int N, int ld, hipStream_t st);
 
 
