@@ -101,8 +101,8 @@ int dh_set_lanczos(dh_system *sys, int steps, double rel_margin);
  * widen it to FP64, so products and sums are FP64 and the only error is the
  * rounding of H_orth itself; 3 = mixed with the FP64 residual correction: mode
  * 1's FP32 products plus the products of the residual H_orth - fl32(H_orth)
- * (stored in bf16, bf16 matrix-core products in accumulators of their own,
- * added in FP64), so the terms apply H_orth to 2^-32 relative instead of
+ * (stored in bf16, bf16 matrix-core products with FP32 accumulators of their
+ * own, added to the main products in FP64), so the terms apply H_orth to 2^-32 relative instead of
  * 2^-24.  Modes 1 and 3 restore each step's FP64 norms.  No counterpart in the
  * reference (all FP64). */
 int dh_set_precision(dh_system *sys, int mode);
