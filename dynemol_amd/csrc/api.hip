@@ -108,6 +108,9 @@ struct dh_system {
     double *lal = nullptr, *lbe = nullptr, *J = nullptr;
     cplx* coef = nullptr;
     int* kout = nullptr;
+    cplx* coef_h = nullptr;         // pinned staging of the coefficients
+    hipEvent_t coef_ev = nullptr;   // the last copy out of coef_h
+    bool coef_pending = false;
     // observables
     double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr, *norms1 = nullptr;
     // host state
@@ -160,6 +163,13 @@ struct dh_system {
         double *lal = nullptr, *lbe = nullptr;
         unsigned* counters = nullptr;
         double bounds[6] = {0, 0, 0, 0, 0, 0};
+        // a launched build's Cholesky status and Lanczos coefficients come back
+        // into pinned host memory; `done` follows them on the side stream
+        int* h_info = nullptr;
+        double *h_al = nullptr, *h_be = nullptr;
+        hipEvent_t done = nullptr;
+        int m = 0;
+        bool pending = false;
         bool allocated = false;
     } sh;
 };
@@ -266,11 +276,17 @@ extern "C" int dh_system_destroy(dh_system* s) {
         if (g.exec) hipGraphExecDestroy(g.exec);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->coef_ev) hipEventDestroy(s->coef_ev);
+    if (s->coef_h) hipHostFree(s->coef_h);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
+    if (s->ctx->side) hipStreamSynchronize(s->ctx->side);  // a launched build may still run
     void* shp[] = {s->sh.Q1, s->sh.Q2, s->sh.Hth, s->sh.coords, s->sh.H32, s->sh.Hlo, s->sh.lv, s->sh.lvp, s->sh.lw,
                    s->sh.partial, s->sh.lal, s->sh.lbe, s->sh.counters};
     for (void* p : shp)
         if (p) hipFree(p);
+    for (void* p : {(void*)s->sh.h_info, (void*)s->sh.h_al, (void*)s->sh.h_be})
+        if (p) hipHostFree(p);
+    if (s->sh.done) hipEventDestroy(s->sh.done);
     void* ptrs[] = {s->basis_mem, s->coords, s->frag, s->Q1, s->Q2, s->Hth, s->phi, s->x0, s->x1, s->acc, s->out, s->stage,
                     s->hk, s->partial, s->counters, s->lv, s->lvp, s->lw, s->lal, s->lbe, s->J, s->coef,
                     s->kout, s->pops, s->erg, s->norms, s->norms0, s->norms1};
@@ -565,7 +581,11 @@ static OpsTarget main_target(dh_system* s) {
 }
 
 // Lanczos spectral bounds of t.Hth into t.bounds (host wait on t.st)
-static int bounds_into(dh_system* s, const OpsTarget& t) {
+// With al_host/be_host (pinned), the coefficients are only copied there on
+// t.st and the host part (lanczos_bounds_host) is left to the caller, after
+// the stream has passed them; *m_out gets their count.
+static int bounds_into(dh_system* s, const OpsTarget& t, double* al_host = nullptr, double* be_host = nullptr,
+                       int* m_out = nullptr) {
     hipStream_t st = t.st;
     const int m = std::min(s->lanczos_steps, std::min(LANCZOS_MAX, s->N));
     launch_lanczos_start(s->N, t.lv, t.lvp, 0x9e3779b9u, st);
@@ -586,6 +606,12 @@ static int bounds_into(dh_system* s, const OpsTarget& t) {
         launch_lanczos_step(s->N, t.lv, t.lvp, t.lw, t.lal, t.lbe, j, st);
     }
     DH_HIP(hipGetLastError());
+    if (al_host) {
+        DH_HIP(hipMemcpyAsync(al_host, t.lal, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        DH_HIP(hipMemcpyAsync(be_host, t.lbe, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, st));
+        *m_out = m;
+        return DH_OK;
+    }
     double al[LANCZOS_MAX], be[LANCZOS_MAX + 1];
     DH_HIP(hipMemcpyAsync(al, t.lal, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     DH_HIP(hipMemcpyAsync(be, t.lbe, sizeof(double) * (m + 1), hipMemcpyDeviceToHost, st));
@@ -695,19 +721,26 @@ static void launch_hprime(dh_system* s, const double* linv) {
 
 // S = L L^T, L^-1, W = L^-1 h, H_orth = W L^-T from t.S and t.h, all on t.st
 // (host wait for the Cholesky status); with hprime (in-place builds only) also
-// H' = L^-T W into the context's Hp.  Q1 and Q2 come back mirrored.
-static int factor_into(dh_system* s, const OpsTarget& t, bool hprime) {
+// H' = L^-T W into the context's Hp.  Q1 and Q2 come back mirrored.  With
+// info_host (pinned) the status is only copied there on t.st and the host
+// does not wait: the caller checks it once the stream has passed it (the
+// products after a failed factorisation are then discarded).
+static int factor_into(dh_system* s, const OpsTarget& t, bool hprime, int* info_host = nullptr) {
     hipStream_t st = t.st;
     const int ld = s->ld;
     launch_copy_matrix(t.S, t.Q1, (long)ld * ld, st);
     DH_HIP(hipMemsetAsync(t.info, 0, sizeof(int), st));
     launch_cholesky(t.Q1, ld, t.Dinv, t.info, st);
-    int info = 0;
-    DH_HIP(hipMemcpyAsync(&info, t.info, sizeof(int), hipMemcpyDeviceToHost, st));
-    DH_HIP(hipStreamSynchronize(st));
-    if (info != 0) {
-        dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
-        return DH_ERR_NOT_SPD;
+    if (info_host) {
+        DH_HIP(hipMemcpyAsync(info_host, t.info, sizeof(int), hipMemcpyDeviceToHost, st));
+    } else {
+        int info = 0;
+        DH_HIP(hipMemcpyAsync(&info, t.info, sizeof(int), hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        if (info != 0) {
+            dh::set_error("dh_orthogonalize: S is not positive definite (Cholesky failed at column %d)", info);
+            return DH_ERR_NOT_SPD;
+        }
     }
     launch_trtri_lower(t.Q1, t.Q2, t.Dinv, ld, t.W, st);  // W is free until W = L^-1 h
     {   // W = L^-1 h
@@ -924,15 +957,22 @@ static int ensure_coefficients(dh_system* s, double dt) {
     const double r = 0.5 * (s->emax - s->emin) * s->widen;
     if (dt == s->coef_dt && c == s->coef_c && r == s->coef_r) return DH_OK;
     hipStream_t st = s->ctx->stream;
-    launch_bessel(r * dt, c * dt, KMAX_ALLOC, BESSEL_TOL, s->J, s->coef, s->kout, st);
-    int k = 0;
-    DH_HIP(hipGetLastError());
-    DH_HIP(hipMemcpyAsync(&k, s->kout, sizeof(int), hipMemcpyDeviceToHost, st));
-    DH_HIP(hipStreamSynchronize(st));
+    // on the host (a few microseconds; the device recurrence is one thread of
+    // dependent memory accesses, about a millisecond), then one copy on the
+    // stream: the host neither launches nor waits for a kernel here
+    if (!s->coef_h) {
+        DH_HIP(hipHostMalloc((void**)&s->coef_h, sizeof(cplx) * (KMAX_ALLOC + 2), hipHostMallocDefault));
+        DH_HIP(hipEventCreateWithFlags(&s->coef_ev, hipEventDisableTiming));
+    }
+    if (s->coef_pending) DH_HIP(hipEventSynchronize(s->coef_ev));  // coef_h is free again
+    const int k = bessel_coef_host(r * dt, c * dt, KMAX_ALLOC, BESSEL_TOL, s->coef_h);
     if (k <= 0 && r * dt > 1e-12) {
         dh::set_error("Chebyshev coefficients: no terms for r*dt = %g", r * dt);
         return DH_ERR_STATE;
     }
+    DH_HIP(hipMemcpyAsync(s->coef, s->coef_h, sizeof(cplx) * (k + 2), hipMemcpyHostToDevice, st));
+    DH_HIP(hipEventRecord(s->coef_ev, st));
+    s->coef_pending = true;
     s->K = k;
     s->coef_dt = dt;
     s->coef_c = c;
@@ -1272,6 +1312,10 @@ static int ensure_side(dh_system* s) {
         DALLOC(h.counters, (size_t)s->ld);
         launch_zero_u32(h.counters, s->ld, c->side);
         launch_zero_cplx(h.lw, s->ld, c->side);
+        DH_HIP(hipHostMalloc((void**)&h.h_info, sizeof(int), hipHostMallocDefault));
+        DH_HIP(hipHostMalloc((void**)&h.h_al, sizeof(double) * (LANCZOS_MAX + 2), hipHostMallocDefault));
+        DH_HIP(hipHostMalloc((void**)&h.h_be, sizeof(double) * (LANCZOS_MAX + 2), hipHostMallocDefault));
+        DH_HIP(hipEventCreateWithFlags(&h.done, hipEventDisableTiming));
         h.allocated = true;
     }
     if (s->precision != 0 && !h.H32) DALLOC(h.H32, mat);
@@ -1281,10 +1325,10 @@ static int ensure_side(dh_system* s) {
 
 // The operators of the geometry at d_geo (device, n_atoms x 3) into the shadow
 // set, on the side stream: S and h, Cholesky, L^-1, H_orth, the precision
-// mode's FP32 copies and the Lanczos bounds.  The host waits on the side stream
-// only, so the main stream's queued steps run meanwhile; the side stream is
-// idle when this returns.
-static int side_build(dh_system* s, const double* d_geo, double cutoff, double s_drop) {
+// mode's FP32 copies and the Lanczos coefficients.  Only launches: the host
+// does not wait here, so it goes on to queue the main stream's steps, which
+// run meanwhile; side_finish completes the build.
+static int side_launch(dh_system* s, const double* d_geo, double cutoff, double s_drop) {
     const auto t0 = std::chrono::steady_clock::now();
     dh_context* c = s->ctx;
     auto& h = s->sh;
@@ -1297,12 +1341,33 @@ static int side_build(dh_system* s, const double* d_geo, double cutoff, double s
     }
     OpsTarget t{c->side, c->S2, c->h2, c->W2, c->Dinv2, c->info2, h.Q1, h.Q2, h.Hth, h.H32, h.Hlo,
                 h.lv, h.lvp, h.lw, h.partial, h.lal, h.lbe, h.counters, h.bounds};
-    int rc = factor_into(s, t, false);
+    int rc = factor_into(s, t, false, h.h_info);
     if (rc) return rc;
     f32_copies_into(s, t);
-    rc = bounds_into(s, t);
+    rc = bounds_into(s, t, h.h_al, h.h_be, &h.m);
+    if (rc) return rc;
+    DH_HIP(hipEventRecord(h.done, c->side));
+    h.pending = true;
     s->stats[8] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return rc;
+    return DH_OK;
+}
+
+// Waits for the launched build (the side stream is idle afterwards), checks its
+// Cholesky status and turns its Lanczos coefficients into the shadow bounds.
+static int side_finish(dh_system* s) {
+    auto& h = s->sh;
+    if (!h.pending) return DH_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    h.pending = false;
+    DH_HIP(hipEventSynchronize(h.done));
+    s->stats[8] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (*h.h_info != 0) {
+        dh::set_error("dh_propagate_steps: S of a rebuilt geometry is not positive definite (Cholesky failed at "
+                      "column %d)", *h.h_info);
+        return DH_ERR_NOT_SPD;
+    }
+    lanczos_bounds_host(h.h_al, h.h_be, h.m, s->rel_margin, h.bounds);
+    return DH_OK;
 }
 
 // The shadow operators become the system's (host pointer swaps; both streams
@@ -1332,11 +1397,14 @@ static void side_commit(dh_system* s) {
     s->stats[7] += 1;
 }
 
-// Device buffers of one dh_propagate_steps call, freed on every return path.
+// Device buffers of one dh_propagate_steps call, freed on every return path
+// (after the side stream, whose launched build reads geo, has drained).
 struct StepsScratch {
     double *geo = nullptr, *n0 = nullptr, *n1 = nullptr, *pops = nullptr, *erg = nullptr;
     cplx *ck_phi = nullptr, *ck_out = nullptr;
+    hipStream_t side = nullptr;
     ~StepsScratch() {
+        if (side) hipStreamSynchronize(side);
         for (void* p : {(void*)geo, (void*)n0, (void*)n1, (void*)pops, (void*)erg, (void*)ck_phi, (void*)ck_out})
             if (p) hipFree(p);
     }
@@ -1366,6 +1434,7 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
     const size_t geo_n = (size_t)s->n_atoms * 3;
     if (n_geo > 0) {
         if ((rc = ensure_side(s))) return rc;
+        d.side = s->ctx->side;
         DALLOC(d.geo, geo_n * n_geo);
         // every geometry goes to the device once; a rebuild reads its slice there
         DH_HIP(hipMemcpyAsync(d.geo, coords, sizeof(double) * geo_n * n_geo, hipMemcpyHostToDevice, st));
@@ -1377,8 +1446,12 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
     for (int j = 0; j < n_steps;) {
         // the operators of this step's geometry were built while the previous
         // interval's steps ran
-        if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) side_commit(s);
+        if (j > 0 && rebuild_every > 0 && j % rebuild_every == 0) {
+            if ((rc = side_finish(s))) return rc;
+            side_commit(s);
+        }
         const int jend = rebuild_every > 0 ? std::min(n_steps, (j / rebuild_every + 1) * rebuild_every) : n_steps;
+        const bool build_next = rebuild_every > 0 && jend < n_steps && jend % rebuild_every == 0;
         if (s->phi_dirty) {
             convert_ao_to_phi(s);
             s->phi_dirty = false;
@@ -1407,15 +1480,15 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                                   hipMemcpyDeviceToDevice, st));
             DH_HIP(hipMemcpyAsync(d.erg + (size_t)i * W * 2, s->erg, sizeof(double) * W * 2, hipMemcpyDeviceToDevice,
                                   st));
-        }
-        DH_HIP(hipGetLastError());
-        // the next geometry's operators, on the side stream, while these steps run
-        if (rebuild_every > 0 && jend < n_steps && jend % rebuild_every == 0) {
-            if ((rc = side_build(s, d.geo + geo_n * (jend / rebuild_every - 1), cutoff, s_drop))) {
-                hipStreamSynchronize(st);
-                return rc;
+            // the next geometry's operators, on the side stream, while this
+            // interval's steps run; launched once the first step is queued so
+            // that the main stream has work meanwhile (the shadow set and the
+            // second workspace hold nothing the main stream still reads)
+            if (i == j && build_next) {
+                if ((rc = side_launch(s, d.geo + geo_n * (jend / rebuild_every - 1), cutoff, s_drop))) return rc;
             }
         }
+        DH_HIP(hipGetLastError());
         const size_t cnt = (size_t)(jend - j) * R;
         DH_HIP(hipMemcpyAsync(h0.data() + (size_t)j * R, d.n0 + (size_t)j * R, sizeof(double) * cnt,
                               hipMemcpyDeviceToHost, st));

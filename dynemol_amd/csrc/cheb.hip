@@ -1101,11 +1101,13 @@ static int sym_seg_for(int nb, int per_cu = 4) {
     return S;
 }
 
-// the partial buffer of both symmetric kernels: the segment count only falls
-// as per_cu falls, so the four-per-CU grid is the largest
+// the partial buffer of the one-row-segment symmetric kernels: the segment
+// count only falls as per_cu falls, so the four-per-CU grid is the largest of
+// the resident grids; the fixed three-tile segments (variant 21) may have more
 long sym_partial_cplx(int ld, int R) {
     const long nb = ld / TILE;
-    return (nb * (nb - 1) / 2 + sym_segment_count((int)nb, sym_seg_for((int)nb, 4))) * TILE * R;
+    const long segs = std::max(sym_segment_count((int)nb, sym_seg_for((int)nb, 4)), sym_segment_count((int)nb, 3));
+    return (nb * (nb - 1) / 2 + segs) * TILE * R;
 }
 
 bool sym_supported(int R, int tri) { return tri == TRI_FULL && (R == 1 || R == 2 || R == 4 || R == 8); }
@@ -1916,14 +1918,146 @@ static void launch_symm2(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((sym2_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
 }
 
+// ------------------------------------------------------------------ symmetric, loads up front
+// Variant 21: the segments, partial layout and 4x4x4 matrix-core tile
+// products of variant 18 (so sym_reduce_kernel is its phase 2), with a fixed
+// segment length T = 3 and every tile and x_J block of the segment loaded
+// into registers before the first product: a workgroup has its whole segment
+// (3 x 32 KB of H) in flight at once instead of one tile ahead, and the grid
+// may exceed the resident workgroups (short-lived workgroups replace each
+// other).  At N = 4096 this is within 2 % of variant 18 (segments of 2 or 4
+// were slower): the term is not bound by the bytes in flight.
+template <int R, int T>
+__global__ __launch_bounds__(256) void symm_up_kernel(const double* __restrict__ Mg, const double* __restrict__ xg,
+                                                      double* __restrict__ part, int ld, int N, int x_stride,
+                                                      int x_off) {
+    constexpr int HP = TILE + 1;
+    constexpr int C2 = 2 * R;
+    constexpr int XE = TILE * C2;
+    constexpr int XPT = (XE + 255) / 256;
+    constexpr int G = (C2 + 3) / 4;
+    __shared__ double Hs[TILE * HP];
+    __shared__ double xs[XE];
+    __shared__ double xis[XE];
+    const int nb = ld / TILE;
+    const int seg = blockIdx.x;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sym_segment_start(mid, T) <= seg) lo = mid;
+        else hi = mid - 1;
+    }
+    const int I = lo;
+    const int Ja = (seg - sym_segment_start(I, T)) * T, Jb = min(I + 1, Ja + T);
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i0 = I * TILE;
+    const int lr = lane & 15, lk = lane >> 4, xq = lane & 3;
+    const long xs2 = 2L * x_stride, xo2 = 2L * x_off;
+    // H's padding is [0 0; 0 I] and x rows >= N are read as row N - 1 (finite
+    // data): rows >= N of x never reach rows < N of the result.
+    double hv[T][16], xv[T][XPT], xi[XPT];
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        if (Ja + u < Jb) {
+            const int j0 = (Ja + u) * TILE;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) hv[u][c] = Mg[(long)(j0 + 4 * c + lk) * ld + i0 + 16 * w + lr];
+#pragma unroll
+            for (int q = 0; q < XPT; ++q) {
+                const int e = t + 256 * q;
+                if (XE % 256 == 0 || e < XE) xv[u][q] = xg[(long)min(j0 + e / C2, N - 1) * xs2 + xo2 + e % C2];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+        const int e = t + 256 * q;
+        if (XE % 256 == 0 || e < XE) xi[q] = xg[(long)min(i0 + e / C2, N - 1) * xs2 + xo2 + e % C2];
+    }
+#pragma unroll
+    for (int q = 0; q < XPT; ++q) {
+        const int e = t + 256 * q;
+        if (XE % 256 == 0 || e < XE) xis[e] = xi[q];
+    }
+    double yacc[G];
+#pragma unroll
+    for (int h = 0; h < G; ++h) yacc[h] = 0.0;
+    // D lane l: row (column) 16w + 4 ((l >> 2) & 3) + (l >> 4) of the tile, real column 4h + (l & 3)
+    const int rowl = 16 * w + 4 * ((lane >> 2) & 3) + lk;
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        const int J = Ja + u;
+        if (J >= Jb) break;  // uniform over the workgroup
+        __syncthreads();     // previous tile's LDS readers are done (x_I staged, first time)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) Hs[(4 * c + lk) * HP + 16 * w + lr] = hv[u][c];
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = t + 256 * q;
+            if (XE % 256 == 0 || e < XE) xs[e] = xv[u][q];
+        }
+        __syncthreads();     // tile and x_J staged
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                yacc[h] = __builtin_amdgcn_mfma_f64_4x4x4f64(hv[u][c], xs[(4 * c + lk) * C2 + min(4 * h + xq, C2 - 1)],
+                                                             yacc[h], 0, 0, 0);
+        if (J == I) continue;  // diagonal tile: the row sums cover it
+        double zacc[G];
+#pragma unroll
+        for (int h = 0; h < G; ++h) zacc[h] = 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const double hz = Hs[(16 * w + lr) * HP + 4 * r + lk];
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                zacc[h] = __builtin_amdgcn_mfma_f64_4x4x4f64(hz, xis[(4 * r + lk) * C2 + min(4 * h + xq, C2 - 1)],
+                                                             zacc[h], 0, 0, 0);
+        }
+        const long tI = (long)I * (I - 1) / 2 + J;
+        double* pz = part + 2 * (tI * TILE * R + (long)rowl * R);
+#pragma unroll
+        for (int h = 0; h < G; ++h)
+            if (4 * h + xq < C2) pz[4 * h + xq] = zacc[h];
+    }
+    const long pcol = (long)nb * (nb - 1) / 2 * TILE * R;
+    double* py = part + 2 * (pcol + (long)seg * TILE * R + (long)rowl * R);
+#pragma unroll
+    for (int h = 0; h < G; ++h)
+        if (4 * h + xq < C2) py[4 * h + xq] = yacc[h];
+}
+
+template <int R, int T>
+static void launch_symm_up(const MatvecArgs& a, hipStream_t st) {
+    const int nb = a.ld / TILE;
+    hipLaunchKernelGGL((symm_up_kernel<R, T>), dim3(sym_segment_count(nb, T)), dim3(256), 0, st, a.M,
+                       reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
+                       a.x_stride, a.x_off);
+    launch_sym_reduce<R>(a, nb, T, st);
+}
+
+constexpr int SYM_UP_T = 3;  // segment length of variant 21
+
+static void launch_symm_up_variant(const MatvecArgs& a, hipStream_t st) {
+    switch (a.R) {
+        case 1: launch_symm_up<1, SYM_UP_T>(a, st); break;
+        case 2: launch_symm_up<2, SYM_UP_T>(a, st); break;
+        case 4: launch_symm_up<4, SYM_UP_T>(a, st); break;
+        default: launch_symm_up<8, SYM_UP_T>(a, st); break;
+    }
+}
+
 // Variant table: 0 = split-K with last-block reduction (needs a.split, partial,
 // counters); 1..4 and 9..14 = column-block kernels of different shapes and LDS
 // schedules; 5..8 = two-phase split-K; 15 = split-K matrix-core GEMM (full
 // operator); 16 = symmetric half storage, vector FMAs; 17 = symmetric half
 // storage, matrix-core tile products (16x16x4); 18 = the same with 4x4x4 products;
 // 19 = 4x4x4 products over 2 x 2 super-tiles; 20 = 4x4x4 products over
-// per-wave column strips, without LDS staging of the tiles.
-constexpr int MV_VARIANTS = 21;
+// per-wave column strips, without LDS staging of the tiles; 21 = the
+// products of 18 over segments of 3 tiles, all loaded up front.
+constexpr int MV_VARIANTS = 22;
 
 template <int R>
 static void launch_mv_r(const MatvecArgs& a, int variant, hipStream_t st) {
@@ -1986,6 +2120,14 @@ void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
+    if (variant == 21) {
+        if (!sym_supported(a.R, a.tri)) {
+            launch_matvec_variant(a, 1, st);
+            return;
+        }
+        launch_symm_up_variant(a, st);
+        return;
+    }
     if (variant == 19) {
         if (!sym_supported(a.R, a.tri)) {
             launch_matvec_variant(a, 1, st);
@@ -2279,6 +2421,54 @@ __global__ void bessel_coef_kernel(double x, double phase, int kmax_alloc, doubl
     }
     coef[K + 1] = cplx{0.0, 0.0};
     out_k[0] = K;
+}
+
+// The same coefficients on the host (the recurrence above, operation for
+// operation): coef[0..K] and coef[K + 1] = 0; returns K.
+int bessel_coef_host(double x, double phase, int kmax_alloc, double tol, cplx* coef) {
+    std::vector<double> J((size_t)kmax_alloc + 2, 0.0);
+    int K = 0;
+    if (x < 1e-12) {
+        J[0] = 1.0;
+    } else {
+        int M = (int)(x + 15.0 * std::cbrt(x) + 40.0);
+        M += M & 1;
+        if (M > kmax_alloc - 2) M = kmax_alloc - 2;
+        double jp1 = 0.0, jk = 1e-300;
+        J[M] = jk;
+        for (int k = M; k >= 1; --k) {
+            const double jm1 = (2.0 * k / x) * jk - jp1;
+            J[k - 1] = jm1;
+            jp1 = jk;
+            jk = jm1;
+            if (std::fabs(jk) > 1e250) {
+                for (int q = k - 1; q <= M; ++q) J[q] *= 1e-250;
+                jk *= 1e-250;
+                jp1 *= 1e-250;
+            }
+        }
+        double norm = J[0];
+        for (int k = 2; k <= M; k += 2) norm += 2.0 * J[k];
+        for (int k = 0; k <= M; ++k) J[k] /= norm;
+        K = M;
+        while (K > 0 && std::fabs(J[K]) < tol) --K;
+        K += 1;
+        if (K > M) K = M;
+    }
+    const double cp = std::cos(phase), sp = -std::sin(phase);
+    for (int k = 0; k <= K; ++k) {
+        const double f = (k == 0 ? 1.0 : 2.0) * J[k];
+        double re, im;
+        switch (k & 3) {
+            case 0: re = f; im = 0.0; break;
+            case 1: re = 0.0; im = -f; break;
+            case 2: re = -f; im = 0.0; break;
+            default: re = 0.0; im = f; break;
+        }
+        coef[k] = cplx{re * cp - im * sp, re * sp + im * cp};
+    }
+    coef[K + 1] = cplx{0.0, 0.0};
+    return K;
 }
 
 // ------------------------------------------------------------------ observables

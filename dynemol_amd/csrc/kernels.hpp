@@ -189,6 +189,7 @@ void launch_lanczos_step(int N, cplx* v, cplx* vprev, const cplx* w, double* alp
 // extreme-eigenvalue bounds of the Lanczos tridiagonal (al[0..m-1], be[1..m]),
 // on the host: bounds = [emin, emax, theta_min, theta_max, res_min, res_max]
 void lanczos_bounds_host(const double* al, const double* be, int m, double rel_margin, double* bounds);
+int bessel_coef_host(double x, double phase, int kmax_alloc, double tol, cplx* coef);
 void launch_bessel(double x, double phase, int kmax_alloc, double tol, double* J, cplx* coef, int* out_k,
                    hipStream_t st);
 void launch_observables(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi, const cplx* hk,

@@ -144,10 +144,12 @@ int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
  * 2r, ... from geometry g = step / r - 1 of coords (n_atoms x 3 per geometry,
  * (n_steps - 1) / r geometries, all copied to the device at the start).  Each
  * rebuild runs on a second stream of the context, into a second set of
- * operator buffers, while the interval before it propagates, and is swapped in
- * at its step (the system and the context then hold about 7 N^2 doubles more).
- * The host waits once per rebuild interval, where the norms of the interval's
- * steps are checked; an interval with a drifting step is redone step by step
+ * operator buffers, while the interval before it propagates (it is launched
+ * before that interval's steps, without host waits), and is swapped in at its
+ * step (the system and the context then hold about 7 N^2 doubles more).  The
+ * host waits once per rebuild interval, where the norms of the interval's
+ * steps and the rebuild's Cholesky status are checked (a failed rebuild
+ * returns DH_ERR_NOT_SPD); an interval with a drifting step is redone step by step
  * with wider bounds.  A rebuild takes the context workspace over, as
  * dh_build_operators does.  pops (n_steps x n_wp x (n_frag + 1)) and erg (n_steps x
  * n_wp x 2) receive each step's populations and energies (may be NULL); the
@@ -181,8 +183,8 @@ int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, dou
 int dh_ehrenfest_kernel(dh_system *sys, int wp_el, int wp_hl, const double *X, int ldx, double *K, int ldk);
 /* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
  * retries, last norm drift, term-series GPU ms (HIP events, when timing is
- * on), term launches timed, operator rebuilds, host ms of the rebuilds that
- * dh_propagate_steps ran beside the propagation]. */
+ * on), term launches timed, operator rebuilds, host ms that dh_propagate_steps
+ * spent launching its rebuilds beside the propagation and waiting for them]. */
 int dh_get_stats(dh_system *sys, double *out9);
 /* Record HIP events around each step's Chebyshev term series (on the
  * context stream that runs them); no counterpart in the reference, whose

@@ -119,7 +119,7 @@ def test_cholesky_reports_non_spd():
     assert info == 71
 
 
-@pytest.mark.parametrize("variant", list(range(21)))
+@pytest.mark.parametrize("variant", list(range(22)))
 @pytest.mark.parametrize("n_wp", [1, 2, 3, 4])
 def test_matvec_variants_propagate_exactly(variant, n_wp):
     """Every matvec kernel variant the autotuner may pick gives the exact propagator."""
@@ -154,12 +154,12 @@ def test_matvec_variants_propagate_exactly(variant, n_wp):
         hip.force_matvec_variant(-1)
 
 
-@pytest.mark.parametrize("variant", [16, 17, 18, 19, 20])
+@pytest.mark.parametrize("variant", [16, 17, 18, 19, 20, 21])
 @pytest.mark.parametrize("n,n_wp", [(1100, 1), (1100, 2), (1100, 4), (1200, 8)])
 def test_symmetric_half_storage_variant_large(n, n_wp, variant):
-    """Variants 16 (vector FMAs), 17 to 20 (matrix-core tile products): lower-triangle
-    tiles, two-phase, over many block rows and segments; even and odd block
-    counts (18 and 19 blocks of 64)."""
+    """Variants 16 (vector FMAs), 17 to 21 (matrix-core tile products): lower-triangle
+    tiles, two-phase, over many block rows and segments (also loaded up front); even and
+    odd block counts (18 and 19 blocks of 64)."""
     rng = np.random.default_rng(n + n_wp)
     _say(f"symmetric variant {variant} N={n} n_wp={n_wp}")
     X = rng.normal(size=(n, n))
