@@ -31,14 +31,16 @@ HBM_PEAK_GBS = 8000.0
 # counter reads for the library's plain copy kernel over 256 MiB (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000): profiles/cheb_term_sym_tio2_N4096_counters.md.
 # Variants 17, 18 and 20 likewise: profiles/cheb_term_symm_tio2_N4096_counters.md,
-# profiles/cheb_term_symm4_tio2_N4096_counters.md and profiles/cheb_term_symm3_tio2_N4096_counters.md.
+# profiles/cheb_term_symm4_tio2_N4096_counters.md and profiles/cheb_term_symm3_tio2_N4096_counters.md;
+# variant 21: profiles/cheb_term_symm_up_N4096_counters.md.
 MEASURED_TRAFFIC_BYTES = {16: int(((35409.8125 + 5487.5) * 2.0 + (10924.0 + 256.0)) * 1024),
                           17: int(((34796.36781939338 + 5874.9525547445255) * 2.0
                                    + (10924.001321231617 + 508.26277372262774)) * 1024),
                           18: int(((34810.57783777574 + 5872.213768115942) * 2.0
                                    + (10924.024931066177 + 506.4347826086956)) * 1024),
                           20: int(((34719.24046415441 + 5869.5) * 2.0
-                                   + (10925.4521484375 + 504.63309352517985)) * 1024)}
+                                   + (10925.4521484375 + 504.63309352517985)) * 1024),
+                          21: int(((34452.90 + 5881.50) * 2.0 + (10924.01 + 512.00)) * 1024)}
 
 
 def log(msg):
@@ -184,7 +186,9 @@ def time_ehrenfest(sysm, reps=5):
             "host_numpy_ms": round(1e3 * cpu_s, 1), "host_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
-SYM_VARIANTS = (16, 17, 18, 19, 20)  # symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4)
+# symmetric half-storage term kernels (vector FMAs; matrix-core 16x16x4, 4x4x4; super-tiles, column
+# strips, segments loaded up front)
+SYM_VARIANTS = (16, 17, 18, 19, 20, 21)
 
 
 def term_bytes(N, R):
@@ -546,7 +550,7 @@ def run_cpu_baseline(st, b, S, h, lumo, homo, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=2000, help="timed nuclear steps (config 2 runs 2000)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
