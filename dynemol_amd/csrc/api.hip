@@ -1894,6 +1894,12 @@ __global__ __launch_bounds__(256) void test_read_kernel(const double2* __restric
     }
 }
 
+extern "C" int dh_test_bessel_coefficients(double x, double phase, double tol, int kmax, double* coef, int* K_out) {
+    CHECK_ARG(x >= 0.0 && tol > 0.0 && kmax >= 4 && coef && K_out, "dh_test_bessel_coefficients: bad argument");
+    *K_out = bessel_coef_host(x, phase, kmax, tol, reinterpret_cast<cplx*>(coef));
+    return DH_OK;
+}
+
 extern "C" int dh_test_read(long long bytes, int repeats, double* ms_per_read) {
     CHECK_ARG(bytes >= 16 && repeats >= 1, "dh_test_read: bad argument");
     const long n2 = (long)(bytes / 16);

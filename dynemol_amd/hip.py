@@ -41,7 +41,7 @@ SYMBOLS = [
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read", "dh_test_device_alloc", "dh_test_device_free",
-                "dh_test_memcpy"]
+                "dh_test_memcpy", "dh_test_bessel_coefficients"]
 
 # Every live context and system is released before interpreter shutdown, while
 # the HIP runtime is still up (a handle freed during finalisation aborts).
@@ -104,6 +104,8 @@ def lib():
     L.dh_test_cholesky.argtypes = [ctypes.c_int, _d, _d, _i]
     L.dh_test_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
     L.dh_test_read.argtypes = [ctypes.c_longlong, ctypes.c_int, _d]
+    L.dh_test_bessel_coefficients.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d,
+                                              ctypes.POINTER(ctypes.c_int)]
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
     L.dh_set_graph.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
@@ -425,6 +427,17 @@ def time_matvec(N, R, variant, reps=20):
     us = ctypes.c_double(0.0)
     _check(lib().dh_test_matvec_timing(N, R, variant, reps, ctypes.byref(us)), "dh_test_matvec_timing")
     return us.value
+
+
+def bessel_coefficients(x, phase=0.0, tol=1e-16, kmax=4096):
+    """The propagation's Chebyshev coefficients (host computation, no device
+    work): complex array of length K + 2 (the last one 0) and K."""
+    out = np.zeros(2 * (kmax + 2))
+    K = ctypes.c_int(0)
+    _check(lib().dh_test_bessel_coefficients(float(x), float(phase), float(tol), int(kmax),
+                                             out.ctypes.data_as(_d), ctypes.byref(K)), "dh_test_bessel_coefficients")
+    c = out[0::2] + 1j * out[1::2]
+    return c[:K.value + 2], K.value
 
 
 def run_read(nbytes, repeats=20):

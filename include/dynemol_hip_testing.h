@@ -33,6 +33,12 @@ int dh_test_force_matvec_variant(int variant);
 int dh_test_device_alloc(long long bytes, void **ptr);
 int dh_test_device_free(void *ptr);
 int dh_test_memcpy(void *dst, const void *src, long long bytes, int to_device);
+/* The Chebyshev coefficients of exp(-i x T) e^{-i phase} as the propagation
+ * computes them (host code, no device work): coef (2 (kmax + 2) doubles,
+ * complex pairs) = (2 - delta_k0) (-i)^k J_k(x) e^{-i phase} for k = 0..K and
+ * 0 at K + 1, where K is one past the last k with |J_k(x)| > tol (at most the
+ * recurrence's start index), and *K_out = K. */
+int dh_test_bessel_coefficients(double x, double phase, double tol, int kmax, double *coef, int *K_out);
 #ifdef __cplusplus
 }
 #endif
