@@ -659,7 +659,9 @@ def main():
                      "dense_equivalent_bytes": term_bytes(N, R),
                      "dense_equivalent_frac": round(term_bytes(N, R) / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4)},
         "propagation_only_value": round(prop_only, 3),
-        "chebyshev_terms_per_step": int(np.median(g["terms"])),
+        # terms launched in the timed region per step (an interval's steps share recurrences)
+        "chebyshev_terms_per_step": round(g["series_launches"] / args.steps, 1),
+        "terms_last_recurrence": int(np.median(g["terms"])),
         # one operator build (S, h, Cholesky, L^-1, H_orth, bounds) on its own; in the timed region
         # the first build of the region runs so and the others beside the propagation
         "rebuild_ms": round(1e3 * g["rebuild_s"], 3),

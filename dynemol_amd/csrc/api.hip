@@ -111,6 +111,12 @@ struct dh_system {
     cplx* coef_h = nullptr;         // pinned staging of the coefficients
     hipEvent_t coef_ev = nullptr;   // the last copy out of coef_h
     bool coef_pending = false;
+    // multi-step series of dh_propagate_steps: MULTI_MAX accumulators and their
+    // coefficient rows (KMAX_ALLOC + 2 each; device, and pinned staging)
+    int steps_per_series = 0;       // 0: as many as the interval allows; 1: one series per step
+    cplx *accm = nullptr, *coefm = nullptr, *coefm_h = nullptr;
+    hipEvent_t coefm_ev = nullptr;
+    bool coefm_pending = false;
     // observables
     double *pops = nullptr, *erg = nullptr, *norms = nullptr, *norms0 = nullptr, *norms1 = nullptr;
     // host state
@@ -278,6 +284,10 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->coef_ev) hipEventDestroy(s->coef_ev);
     if (s->coef_h) hipHostFree(s->coef_h);
+    if (s->coefm_ev) hipEventDestroy(s->coefm_ev);
+    if (s->coefm_h) hipHostFree(s->coefm_h);
+    if (s->accm) hipFree(s->accm);
+    if (s->coefm) hipFree(s->coefm);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     if (s->ctx->side) hipStreamSynchronize(s->ctx->side);  // a launched build may still run
     void* shp[] = {s->sh.Q1, s->sh.Q2, s->sh.Hth, s->sh.coords, s->sh.H32, s->sh.Hlo, s->sh.lv, s->sh.lvp, s->sh.lw,
@@ -997,6 +1007,15 @@ struct SeriesOps {
     LibHandle* lib;             // precision modes 1 and 3
     int prec, K;
     double c, r;
+    // a multi-step series (m > 0): accumulator q at accm + q * accm_stride takes
+    // the coefficients at coefm + q * coefm_stride, terms 0..Kq[q] (ascending);
+    // K is then Kq[m - 1] and acc / coef are not used
+    int m = 0;
+    cplx* accm = nullptr;
+    long accm_stride = 0;
+    const cplx* coefm = nullptr;
+    int coefm_stride = 0;
+    const int* Kq = nullptr;
 };
 
 
@@ -1017,6 +1036,14 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
     a.acc = o.acc;
     a.coef = o.coef;
     a.M32 = o.H32;
+    if (o.m > 0) {
+        a.accm = o.accm;
+        a.accm_stride = o.accm_stride;
+        a.coefm = o.coefm;
+        a.coefm_stride = o.coefm_stride;
+        a.m_lo = 0;
+        a.m_hi = o.m;
+    }
     const int prec = o.prec;
     // modes 1 and 3: the FP32 products read FP32 copies of their input (and mode
     // 3's residual products bf16 copies); the step's state is converted once,
@@ -1057,6 +1084,8 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
     cplx* prev = o.x1;
     for (int k = 2; k <= o.K; ++k) {
         cplx* outb = (k == 2) ? o.x0 : const_cast<cplx*>(prev2);
+        if (o.m > 0)  // accumulators whose series has ended take no more terms
+            while (a.m_lo < o.m - 1 && o.Kq[a.m_lo] < k) ++a.m_lo;
         a.epi = EPI_CHEB;
         a.x = prev;
         a.x2 = prev2;
@@ -1397,6 +1426,91 @@ static void side_commit(dh_system* s) {
     s->stats[7] += 1;
 }
 
+// Steps of one multi-step series starting now, at most `left`: the system's
+// steps_per_series (0: MULTI_MAX), within the coefficient table (the Bessel
+// recurrence of r m dt terms must start below KMAX_ALLOC).  1 means a series
+// per step.
+static int multi_chunk_steps(const dh_system* s, double dt, int left) {
+    const int cap = s->steps_per_series > 0 ? std::min(s->steps_per_series, MULTI_MAX) : MULTI_MAX;
+    int m = std::min(left, cap);
+    const double r = 0.5 * (s->emax - s->emin) * s->widen;
+    while (m > 1) {
+        const double x = r * m * dt;
+        if (x + 15.0 * std::cbrt(x) + 40.0 < KMAX_ALLOC - 2) break;
+        --m;
+    }
+    return std::max(m, 1);
+}
+
+// m steps of dt from s->phi in one Chebyshev recurrence: the terms T_k(Hh) phi
+// do not depend on time, so step q's state sum_k a_k((q + 1) dt) T_k phi
+// collects in accumulator q of s->accm with its own coefficients, all from the
+// same T_k.  The series runs to the last step's K; accumulator q stops taking
+// terms after its own K_q.  Results equal m single steps to rounding (the
+// propagator is exact in both), with about r^(1/3) (m - m^(1/3)) fewer terms
+// than m series of one step.  Returns the terms of the series in *kmax.
+static int multi_series(dh_system* s, double dt, int m, int* kmax) {
+    hipStream_t st = s->ctx->stream;
+    const int R = 2 * s->W;
+    const long acc_n = (long)s->ld * R;
+    constexpr int CS = KMAX_ALLOC + 2;
+    if (!s->accm) {
+        DALLOC(s->accm, (size_t)MULTI_MAX * acc_n);
+        DALLOC(s->coefm, (size_t)MULTI_MAX * CS);
+        DH_HIP(hipHostMalloc((void**)&s->coefm_h, sizeof(cplx) * MULTI_MAX * CS, hipHostMallocDefault));
+        DH_HIP(hipEventCreateWithFlags(&s->coefm_ev, hipEventDisableTiming));
+    }
+    if (s->coefm_pending) DH_HIP(hipEventSynchronize(s->coefm_ev));  // coefm_h is free again
+    const double c = 0.5 * (s->emax + s->emin);
+    const double r = 0.5 * (s->emax - s->emin) * s->widen;
+    int Kq[MULTI_MAX];
+    for (int q = 0; q < m; ++q) {
+        cplx* row = s->coefm_h + (size_t)q * CS;
+        const int k = bessel_coef_host(r * (q + 1) * dt, c * (q + 1) * dt, KMAX_ALLOC, BESSEL_TOL, row);
+        if (k <= 0 && r * dt > 1e-12) {
+            dh::set_error("Chebyshev coefficients: no terms for r*dt = %g", r * (q + 1) * dt);
+            return DH_ERR_STATE;
+        }
+        Kq[q] = k;
+    }
+    // the accumulators leave the series in order: K_q non-decreasing in q, a
+    // shorter row padded with zero coefficients (adding 0 T_k changes nothing)
+    for (int q = 1; q < m; ++q)
+        if (Kq[q] < Kq[q - 1]) {
+            cplx* row = s->coefm_h + (size_t)q * CS;
+            for (int k = Kq[q] + 1; k <= Kq[q - 1] + 1; ++k) row[k] = cplx{0.0, 0.0};
+            Kq[q] = Kq[q - 1];
+        }
+    DH_HIP(hipMemcpyAsync(s->coefm, s->coefm_h, sizeof(cplx) * (size_t)m * CS, hipMemcpyHostToDevice, st));
+    DH_HIP(hipEventRecord(s->coefm_ev, st));
+    s->coefm_pending = true;
+    SeriesOps o{};
+    o.H = s->Hth;
+    o.ld = s->ld;
+    o.N = s->N;
+    o.R = R;
+    o.phi = s->phi;
+    o.x0 = s->x0;
+    o.x1 = s->x1;
+    o.partial = s->partial;
+    o.counters = s->counters;
+    o.lib = s->ctx->lib;
+    o.prec = 0;
+    o.c = c;
+    o.r = r;
+    o.m = m;
+    o.accm = s->accm;
+    o.accm_stride = acc_n;
+    o.coefm = s->coefm;
+    o.coefm_stride = CS;
+    o.Kq = Kq;
+    o.K = Kq[m - 1];
+    if (int rc = series_core(o, st)) return rc;
+    DH_HIP(hipGetLastError());
+    *kmax = Kq[m - 1];
+    return DH_OK;
+}
+
 // Device buffers of one dh_propagate_steps call, freed on every return path
 // (after the side stream, whose launched build reads geo, has drained).
 struct StepsScratch {
@@ -1466,27 +1580,70 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                 DH_HIP(hipEventCreate(&e));
                 s->evpool.push_back(e);
             }
-        for (int i = j; i < jend; ++i) {
-            launch_column_norms(s->N, R, s->phi, d.n0 + (size_t)i * R, st);
-            if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * (i - j)], st));
-            if ((rc = chebyshev_series(s))) return rc;
-            if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * (i - j) + 1], st));
-            std::swap(s->phi, s->acc);
-            launch_column_norms(s->N, R, s->phi, d.n1 + (size_t)i * R, st);
-            if (fp32_products(s))
-                launch_renorm_columns(s->phi, s->N, R, d.n0 + (size_t)i * R, d.n1 + (size_t)i * R, st);
-            launch_outputs(s);
-            DH_HIP(hipMemcpyAsync(d.pops + (size_t)i * W * NP, s->pops, sizeof(double) * W * NP,
-                                  hipMemcpyDeviceToDevice, st));
-            DH_HIP(hipMemcpyAsync(d.erg + (size_t)i * W * 2, s->erg, sizeof(double) * W * 2, hipMemcpyDeviceToDevice,
-                                  st));
+        // FP64 steps through a kernel whose epilogue takes several accumulators
+        // run as multi-step series (up to MULTI_MAX steps per recurrence)
+        const bool multi_ok = s->precision == 0 && s->steps_per_series != 1 &&
+                              matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL));
+        int n_timed = 0, chunk_terms = 0;
+        long terms_timed = 0, launches = 0;
+        for (int i = j; i < jend;) {
+            const int mc = multi_ok ? multi_chunk_steps(s, dt, jend - i) : 1;
+            if (mc >= 2) {
+                if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * n_timed], st));
+                int kmax = 0;
+                if ((rc = multi_series(s, dt, mc, &kmax))) return rc;
+                if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * n_timed + 1], st));
+                if (s->timing) {
+                    ++n_timed;
+                    terms_timed += kmax;
+                }
+                launches += kmax + 3L * mc;
+                chunk_terms = kmax + 1;
+                const long acc_n = (long)s->ld * R;
+                cplx* keep = s->phi;
+                for (int q = 0; q < mc; ++q) {
+                    cplx* st_q = s->accm + (size_t)q * acc_n;
+                    launch_column_norms(s->N, R, q == 0 ? keep : st_q - acc_n, d.n0 + (size_t)(i + q) * R, st);
+                    launch_column_norms(s->N, R, st_q, d.n1 + (size_t)(i + q) * R, st);
+                    s->phi = st_q;  // the outputs of step i + q
+                    launch_outputs(s);
+                    s->phi = keep;
+                    DH_HIP(hipMemcpyAsync(d.pops + (size_t)(i + q) * W * NP, s->pops, sizeof(double) * W * NP,
+                                          hipMemcpyDeviceToDevice, st));
+                    DH_HIP(hipMemcpyAsync(d.erg + (size_t)(i + q) * W * 2, s->erg, sizeof(double) * W * 2,
+                                          hipMemcpyDeviceToDevice, st));
+                }
+                DH_HIP(hipMemcpyAsync(s->phi, s->accm + (size_t)(mc - 1) * acc_n, sizeof(cplx) * acc_n,
+                                      hipMemcpyDeviceToDevice, st));
+            } else {
+                launch_column_norms(s->N, R, s->phi, d.n0 + (size_t)i * R, st);
+                if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * n_timed], st));
+                if ((rc = chebyshev_series(s))) return rc;
+                if (s->timing) DH_HIP(hipEventRecord(s->evpool[2 * n_timed + 1], st));
+                if (s->timing) {
+                    ++n_timed;
+                    terms_timed += s->K;
+                }
+                launches += s->K + 3L;
+                chunk_terms = s->K + 1;
+                std::swap(s->phi, s->acc);
+                launch_column_norms(s->N, R, s->phi, d.n1 + (size_t)i * R, st);
+                if (fp32_products(s))
+                    launch_renorm_columns(s->phi, s->N, R, d.n0 + (size_t)i * R, d.n1 + (size_t)i * R, st);
+                launch_outputs(s);
+                DH_HIP(hipMemcpyAsync(d.pops + (size_t)i * W * NP, s->pops, sizeof(double) * W * NP,
+                                      hipMemcpyDeviceToDevice, st));
+                DH_HIP(hipMemcpyAsync(d.erg + (size_t)i * W * 2, s->erg, sizeof(double) * W * 2,
+                                      hipMemcpyDeviceToDevice, st));
+            }
             // the next geometry's operators, on the side stream, while this
-            // interval's steps run; launched once the first step is queued so
+            // interval's steps run; launched once the first series is queued so
             // that the main stream has work meanwhile (the shadow set and the
             // second workspace hold nothing the main stream still reads)
             if (i == j && build_next) {
                 if ((rc = side_launch(s, d.geo + geo_n * (jend / rebuild_every - 1), cutoff, s_drop))) return rc;
             }
+            i += mc;
         }
         DH_HIP(hipGetLastError());
         const size_t cnt = (size_t)(jend - j) * R;
@@ -1497,7 +1654,7 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
         DH_HIP(hipStreamSynchronize(st));
         float series_ms = 0.0f;
         if (s->timing)
-            for (int i = 0; i < jend - j; ++i) {
+            for (int i = 0; i < n_timed; ++i) {
                 float ms = 0.0f;
                 DH_HIP(hipEventElapsedTime(&ms, s->evpool[2 * i], s->evpool[2 * i + 1]));
                 series_ms += ms;
@@ -1509,14 +1666,14 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
         }
         if (drift < drift_tol) {
             s->stats[4] = drift;
-            s->stats[0] = s->K + 1;
+            s->stats[0] = chunk_terms;
             s->stats[1] += jend - j;
-            s->stats[2] += (double)(s->K + 3) * (jend - j);
+            s->stats[2] += (double)launches;
             if (s->timing) {
                 s->stats[5] += series_ms;
-                s->stats[6] += (double)s->K * (jend - j);
+                s->stats[6] += (double)terms_timed;
             }
-            last_terms = s->K + 1;
+            last_terms = chunk_terms;
         } else {
             // a step left the bounds: back to the interval's start, then step by
             // step with the checked step (wider bounds where needed)
@@ -1571,6 +1728,12 @@ extern "C" int dh_get_stats(dh_system* s, double* out9) {
 extern "C" int dh_set_timing(dh_system* s, int on) {
     CHECK_ARG(s, "dh_set_timing: null system");
     s->timing = on != 0;
+    return DH_OK;
+}
+
+extern "C" int dh_set_steps_per_series(dh_system* s, int m) {
+    CHECK_ARG(s && m >= 0, "dh_set_steps_per_series: need a system and m >= 0");
+    s->steps_per_series = m;
     return DH_OK;
 }
 

@@ -710,10 +710,61 @@ __host__ __device__ inline unsigned short bf16_rne(float f) {
     return (unsigned short)(u >> 16);
 }
 
+// The accumulators of a multi-step series (a.accm): term k adds
+// coefm[q][k] T_k to accumulator q (each with its own time's coefficients, the
+// same T_k); the first term sets them to coefm[q][0] x + coefm[q][1] T_1.
+// am holds accumulators m_lo .. m_hi - 1 as loaded before (EPI_CHEB); all are
+// read before any is written.
+__device__ __forceinline__ void multi_load(const MatvecArgs& a, long oi, cplx* am) {
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u)
+        if (a.m_lo + u < a.m_hi) am[u] = a.accm[(long)(a.m_lo + u) * a.accm_stride + oi];
+}
+
+// the coefficients a_k of accumulators m_lo .. m_hi - 1 for this term (EPI_CHEB)
+__device__ __forceinline__ void multi_coefs(const MatvecArgs& a, cplx* cm) {
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u)
+        if (a.m_lo + u < a.m_hi) cm[u] = a.coefm[(long)(a.m_lo + u) * a.coefm_stride + a.k];
+}
+
+// cm: the coefficients as multi_coefs loads them (EPI_CHEB; registers or LDS),
+// read before the first store; EPI_CHEB_FIRST (one term per series) reads its
+// two coefficients per accumulator directly
+__device__ __forceinline__ void multi_store(const MatvecArgs& a, long oi, cplx xv, cplx T, const cplx* am,
+                                            const cplx* cm) {
+    if (a.epi == EPI_CHEB) {
+#pragma unroll
+        for (int u = 0; u < MULTI_MAX; ++u) {
+            const int q = a.m_lo + u;
+            if (q < a.m_hi) {
+                const cplx add = cmul(cm[u], T);
+                a.accm[(long)q * a.accm_stride + oi] = cplx{am[u].re + add.re, am[u].im + add.im};
+            }
+        }
+    } else {
+        for (int q = a.m_lo; q < a.m_hi; ++q) {
+            const cplx* cq = a.coefm + (long)q * a.coefm_stride;
+            const cplx t0 = cmul(cq[0], xv), t1 = cmul(cq[1], T);
+            a.accm[(long)q * a.accm_stride + oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+        }
+    }
+}
+
+__device__ __forceinline__ void multi_accumulate(const MatvecArgs& a, long oi, cplx xv, cplx T) {
+    cplx am[MULTI_MAX], cm[MULTI_MAX];
+    if (a.epi == EPI_CHEB) {
+        multi_load(a, oi, am);
+        multi_coefs(a, cm);
+    }
+    multi_store(a, oi, xv, T, am, cm);
+}
+
 // Chebyshev epilogue for output row i, column r with y = (H x)_ir:
 //   EPI_PLAIN:       out = y
 //   EPI_CHEB_FIRST:  T = alpha y + beta x; acc = a_0 x + a_1 T; out = T
 //   EPI_CHEB:        T = alpha y + beta x - x2; acc += a_k T; out = T
+// (with a.accm, the accumulators of the multi-step series instead of acc)
 __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r, cplx y) {
     const long oi = (long)i * a.out_stride + a.out_off + r;
     if (a.epi == EPI_PLAIN) {
@@ -726,6 +777,10 @@ __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r,
         const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
         T.re -= x2.re;
         T.im -= x2.im;
+    }
+    if (a.accm) {
+        multi_accumulate(a, oi, xv, T);
+    } else if (a.epi == EPI_CHEB) {
         const cplx add = cmul(a.coef[a.k], T);
         cplx acc = a.acc[oi];
         acc.re += add.re;
@@ -758,10 +813,58 @@ __device__ __forceinline__ EpiOperands epilogue_operands(const MatvecArgs& a, in
     o.xv = a.x[(long)i * a.x_stride + a.x_off + r];
     if (a.epi == EPI_CHEB) {
         o.x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
-        o.acc = a.acc[oi];
-        o.ck = a.coef[a.k];
+        if (!a.accm) {
+            o.acc = a.acc[oi];
+            o.ck = a.coef[a.k];
+        }
     }
     return o;
+}
+
+// T_k of the epilogue (EPI_CHEB or EPI_CHEB_FIRST) from y and its operands
+__device__ __forceinline__ cplx term_value_pre(const MatvecArgs& a, cplx y, const EpiOperands& o) {
+    cplx T{a.alpha * y.re + a.beta * o.xv.re, a.alpha * y.im + a.beta * o.xv.im};
+    if (a.epi == EPI_CHEB) {
+        T.re -= o.x2.re;
+        T.im -= o.x2.im;
+    }
+    return T;
+}
+
+// Accumulator updates of a multi-step series spread over the workgroup:
+// group g (64 threads, one element each) owns accumulators m_lo + g + v G,
+// v < QP, prefetched with their coefficients at the start (multi_spread_load)
+// and updated with T_k from LDS at the end (multi_spread_store), so that no
+// thread updates more than QP of them.
+template <int G>
+struct MultiSpread {
+    static constexpr int QP = (MULTI_MAX + G - 1) / G;
+    cplx am[QP], cq[QP];
+};
+
+template <int G>
+__device__ __forceinline__ void multi_spread_load(const MatvecArgs& a, int g, long oi, bool valid, MultiSpread<G>& ms) {
+#pragma unroll
+    for (int v = 0; v < MultiSpread<G>::QP; ++v) {
+        const int q = a.m_lo + g + v * G;
+        if (valid && q < a.m_hi) {
+            ms.am[v] = a.accm[(long)q * a.accm_stride + oi];
+            ms.cq[v] = a.coefm[(long)q * a.coefm_stride + a.k];
+        }
+    }
+}
+
+template <int G>
+__device__ __forceinline__ void multi_spread_store(const MatvecArgs& a, int g, long oi, bool valid,
+                                                   const MultiSpread<G>& ms, cplx T) {
+#pragma unroll
+    for (int v = 0; v < MultiSpread<G>::QP; ++v) {
+        const int q = a.m_lo + g + v * G;
+        if (valid && q < a.m_hi) {
+            const cplx add = cmul(ms.cq[v], T);
+            a.accm[(long)q * a.accm_stride + oi] = cplx{ms.am[v].re + add.re, ms.am[v].im + add.im};
+        }
+    }
 }
 
 __device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, int r, cplx y, const EpiOperands& o) {
@@ -774,6 +877,8 @@ __device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, in
     if (a.epi == EPI_CHEB) {
         T.re -= o.x2.re;
         T.im -= o.x2.im;
+    }
+    if (a.epi == EPI_CHEB) {
         const cplx add = cmul(o.ck, T);
         a.acc[oi] = cplx{o.acc.re + add.re, o.acc.im + add.im};
     } else {
@@ -1274,9 +1379,10 @@ constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per sourc
 static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
 __host__ __device__ inline int sym_reduce_groups(int nb) { return nb > 32 ? 16 : 4; }
 
-template <int R, int G>
+template <int R, int G, bool MULTI = false>
 __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int chunk) {
     __shared__ cplx red[G * 64];
+    __shared__ cplx Ts[MULTI ? 64 : 1];
     const int nb = a.ld / TILE;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
     const int seg0 = sym_segment_start(B, S);
@@ -1285,12 +1391,15 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     const int s0 = g * L / G, s1 = (g + 1) * L / G;
     const long E = (long)TILE * R;
     const long pcol = (long)nb * (nb - 1) / 2 * E;
+    const int i = B * TILE + e / R;
+    const long oi = (long)i * a.out_stride + a.out_off + e % R;
     // the epilogue's operands do not depend on the partials: fetch them first
+    // (a multi-step series' accumulators: every group its own share)
     EpiOperands ep{};
-    if (t < 64) {
-        const int i = B * TILE + e / R;
-        if (i < a.N) ep = epilogue_operands(a, i, e % R);
-    }
+    if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
+    MultiSpread<G> ms;
+    const bool spread = MULTI && a.epi == EPI_CHEB;
+    if (spread) multi_spread_load<G>(a, g, oi, i < a.N, ms);
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -1310,28 +1419,44 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     }
     red[t] = cplx{sr, si};
     __syncthreads();
-    if (t >= 64) return;
-    cplx y{0.0, 0.0};
+    if (t < 64) {
+        cplx y{0.0, 0.0};
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg) {
-        y.re += red[gg * 64 + t].re;
-        y.im += red[gg * 64 + t].im;
+        for (int gg = 0; gg < G; ++gg) {
+            y.re += red[gg * 64 + t].re;
+            y.im += red[gg * 64 + t].im;
+        }
+        if (!MULTI) {
+            if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
+        } else if (i < a.N) {
+            const cplx T = term_value_pre(a, y, ep);
+            a.out[oi] = T;
+            Ts[t] = T;
+            if (a.epi == EPI_CHEB_FIRST) multi_store(a, oi, ep.xv, T, nullptr, nullptr);
+        }
     }
-    const int i = B * TILE + e / R;
-    if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
+    if (spread) {
+        __syncthreads();  // T_k of the 64 elements in LDS
+        multi_spread_store<G>(a, g, oi, i < a.N, ms, Ts[t & 63]);
+    }
 }
 
-template <int R, int G>
+// MULTI: the epilogue of a multi-step series (a.accm), compiled apart so that
+// the other launches keep their registers
+template <int R, int G, bool MULTI>
 __global__ __launch_bounds__(G * 64) void sym_reduce_kernel(MatvecArgs a, int S) {
-    sym_reduce_body<R, G>(a, S, blockIdx.x, blockIdx.y);
+    sym_reduce_body<R, G, MULTI>(a, S, blockIdx.x, blockIdx.y);
 }
 
 template <int R>
 static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st) {
-    if (sym_reduce_groups(nb) == 16)
-        hipLaunchKernelGGL((sym_reduce_kernel<R, 16>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
-    else
-        hipLaunchKernelGGL((sym_reduce_kernel<R, 4>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
+    if (sym_reduce_groups(nb) == 16) {
+        if (a.accm) hipLaunchKernelGGL((sym_reduce_kernel<R, 16, true>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
+        else hipLaunchKernelGGL((sym_reduce_kernel<R, 16, false>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
+    } else {
+        if (a.accm) hipLaunchKernelGGL((sym_reduce_kernel<R, 4, true>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
+        else hipLaunchKernelGGL((sym_reduce_kernel<R, 4, false>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
+    }
 }
 
 // Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),
@@ -1861,9 +1986,10 @@ __global__ __launch_bounds__(256) void symm2_tiles_kernel(const double* __restri
 // Phase 2 of variant 19: block B adds the row partials of super-tiles
 // (B / 2, 0..B / 2) (row B % 2 of each) and the column partials Pcol[P^2 + B]
 // for the super-tile rows P below B, in a fixed order, then the epilogue.
-template <int R>
+template <int R, bool MULTI>
 __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecArgs a) {
     __shared__ cplx red[SYM_G * 64];
+    __shared__ cplx Ts[MULTI ? 64 : 1];
     const int nb = a.ld / TILE, npr = (nb + 1) / 2;
     const int B = blockIdx.x, chunk = blockIdx.y;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
@@ -1874,11 +2000,13 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
     const int nr = P + 1;
     const int L = nr + (npr - plo);
     const int s0 = g * L / SYM_G, s1 = (g + 1) * L / SYM_G;
+    const int i = B * TILE + e / R;
+    const long oi = (long)i * a.out_stride + a.out_off + e % R;
     EpiOperands ep{};
-    if (t < 64) {
-        const int i = B * TILE + e / R;
-        if (i < a.N) ep = epilogue_operands(a, i, e % R);
-    }
+    if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
+    MultiSpread<SYM_G> ms;
+    const bool spread = MULTI && a.epi == EPI_CHEB;
+    if (spread) multi_spread_load<SYM_G>(a, g, oi, i < a.N, ms);
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -1898,15 +2026,26 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
     }
     red[t] = cplx{sr, si};
     __syncthreads();
-    if (t >= 64) return;
-    cplx y{0.0, 0.0};
+    if (t < 64) {
+        cplx y{0.0, 0.0};
 #pragma unroll
-    for (int gg = 0; gg < SYM_G; ++gg) {
-        y.re += red[gg * 64 + t].re;
-        y.im += red[gg * 64 + t].im;
+        for (int gg = 0; gg < SYM_G; ++gg) {
+            y.re += red[gg * 64 + t].re;
+            y.im += red[gg * 64 + t].im;
+        }
+        if (!MULTI) {
+            if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
+        } else if (i < a.N) {
+            const cplx T = term_value_pre(a, y, ep);
+            a.out[oi] = T;
+            Ts[t] = T;
+            if (a.epi == EPI_CHEB_FIRST) multi_store(a, oi, ep.xv, T, nullptr, nullptr);
+        }
     }
-    const int i = B * TILE + e / R;
-    if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
+    if (spread) {
+        __syncthreads();  // T_k of the 64 elements in LDS
+        multi_spread_store<SYM_G>(a, g, oi, i < a.N, ms, Ts[t & 63]);
+    }
 }
 
 template <int R>
@@ -1915,7 +2054,8 @@ static void launch_symm2(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((symm2_tiles_kernel<R>), dim3(npr * (npr + 1) / 2), dim3(256), 0, st, a.M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off);
-    hipLaunchKernelGGL((sym2_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
+    if (a.accm) hipLaunchKernelGGL((sym2_reduce_kernel<R, true>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
+    else hipLaunchKernelGGL((sym2_reduce_kernel<R, false>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
 }
 
 // ------------------------------------------------------------------ symmetric, loads up front
@@ -2118,6 +2258,11 @@ void matvec_set_tuned(int N, int R, int tri, int variant) {
 }
 void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
+
+// variants whose epilogue is term_epilogue or term_epilogue_pre (the split-K
+// GEMM and the symmetric half-storage kernels), which take a multi-step
+// series' accumulators
+bool matvec_variant_multi_ok(int variant) { return variant >= 15 && variant <= 21; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
     if (variant == 21) {

@@ -144,7 +144,19 @@ struct MatvecArgs {
     const float* x32;       // mixed mode: FP32 copy of x (same layout), read by the FP32 GEMM; may be null
     float* out32;           // mixed mode: the epilogue also writes T_k here in FP32 (same layout); may be null
     unsigned short* outb16; // precision mode 3: the epilogue also writes T_k here in bf16 (same layout); may be null
+    // Several time steps from one recurrence (EPI_CHEB_FIRST / EPI_CHEB): with
+    // accm set, accumulator q (layout of acc, at accm + q * accm_stride) takes
+    // coefm[q * coefm_stride + k] T_k for q in [m_lo, m_hi), in place of acc
+    // and coef; the kernels whose epilogue is term_epilogue or
+    // term_epilogue_pre support it (matvec_variant_multi_ok).
+    cplx* accm;
+    long accm_stride;
+    const cplx* coefm;
+    int coefm_stride;
+    int m_lo, m_hi;
 };
+constexpr int MULTI_MAX = 10;  // accumulators of one multi-step series
+bool matvec_variant_multi_ok(int variant);
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
 long sym_partial_cplx(int ld, int R);

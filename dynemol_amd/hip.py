@@ -33,7 +33,7 @@ SYMBOLS = [
     "dh_last_error", "dh_version", "dh_init", "dh_finalize", "dh_system_create", "dh_system_destroy",
     "dh_system_bytes", "dh_system_bytes_prec", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
-    "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_matvec_tuning", "dh_set_precision",
+    "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_set_steps_per_series", "dh_matvec_tuning", "dh_set_precision",
     "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
     "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
     "dh_chebyshev_work_bytes", "dh_chebyshev_apply", "dh_chebyshev_apply_noalloc", "dh_propagate_steps",
@@ -107,6 +107,7 @@ def lib():
     L.dh_test_bessel_coefficients.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d,
                                               ctypes.POINTER(ctypes.c_int)]
     L.dh_set_timing.argtypes = [_vp, ctypes.c_int]
+    L.dh_set_steps_per_series.argtypes = [_vp, ctypes.c_int]
     L.dh_set_graph.argtypes = [_vp, ctypes.c_int]
     L.dh_matvec_tuning.argtypes = [_vp, _i]
     L.dh_set_precision.argtypes = [_vp, ctypes.c_int]
@@ -359,6 +360,10 @@ class System:
 
     def set_timing(self, on=True):
         _check(lib().dh_set_timing(self.h, 1 if on else 0), "dh_set_timing")
+
+    def set_steps_per_series(self, m):
+        """Steps per Chebyshev recurrence in propagate_steps (0: automatic, 1: one per step)."""
+        _check(lib().dh_set_steps_per_series(self.h, int(m)), "dh_set_steps_per_series")
 
     def set_graph(self, on=True):
         """Term series as one captured hipGraph per step (default) or as separate launches."""

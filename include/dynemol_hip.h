@@ -153,9 +153,11 @@ int dh_propagate(dh_system *sys, double t_init, double t_max, int *n_terms);
  * with wider bounds.  A rebuild takes the context workspace over, as
  * dh_build_operators does.  pops (n_steps x n_wp x (n_frag + 1)) and erg (n_steps x
  * n_wp x 2) receive each step's populations and energies (may be NULL); the
- * final state is read as after dh_propagate.  Same kernels and results as
- * n_steps calls of dh_propagate(sys, 0, dt) with dh_set_coords +
- * dh_build_operators at the rebuild steps.
+ * final state is read as after dh_propagate.  With dh_set_steps_per_series(sys,
+ * 1): the same kernels and results as n_steps calls of dh_propagate(sys, 0, dt)
+ * with dh_set_coords + dh_build_operators at the rebuild steps; by default FP64
+ * steps share recurrences (dh_set_steps_per_series) and agree with those to
+ * rounding.
  */
 int dh_propagate_steps(dh_system *sys, int n_steps, double dt, int rebuild_every, const double *coords,
                        double cutoff, double s_drop, double *pops, double *erg, int *n_terms);
@@ -181,7 +183,8 @@ int dh_get_observables(dh_system *sys, int wp, double *ao_bra, double *pops, dou
  * calculate_rho / Huckel_stuff steps before it (diabatic-Ehren.f:100-116,
  * 250-276, 366-392); the host copies of H', A and X it needed are gone. */
 int dh_ehrenfest_kernel(dh_system *sys, int wp_el, int wp_hl, const double *X, int ldx, double *K, int ldk);
-/* Counters (9 doubles): [terms of last step, steps, matvec launches, divergence
+/* Counters (9 doubles): [terms of the last recurrence (one step's, or a
+ * multi-step series' in dh_propagate_steps), steps, matvec launches, divergence
  * retries, last norm drift, term-series GPU ms (HIP events, when timing is
  * on), term launches timed, operator rebuilds, host ms that dh_propagate_steps
  * spent launching its rebuilds beside the propagation and waiting for them]. */
@@ -190,6 +193,18 @@ int dh_get_stats(dh_system *sys, double *out9);
  * context stream that runs them); no counterpart in the reference, whose
  * only timing hook is the GPU_TIMING wall-clock print (GPU_Interface.cpp:33-49). */
 int dh_set_timing(dh_system *sys, int on);
+/* Steps per Chebyshev recurrence in dh_propagate_steps (FP64 systems whose
+ * autotuned term kernel is the split-K GEMM or a symmetric half-storage
+ * kernel): within a rebuild interval the operator is fixed and the terms
+ * T_k(Hh) phi do not depend on time, so one recurrence to the last step's K
+ * collects every step's state, each with its own coefficients.  m = 0
+ * (default): up to 10 steps per recurrence, within the interval and the
+ * 4096-term coefficient table; m = 1: one recurrence per step, the same
+ * kernels and bits as dh_propagate.  The steps' states agree to rounding
+ * (the propagator is exact either way) with fewer terms: at N = 4096 (TiO2
+ * slab) 10 steps take about 2110 terms instead of 10 x 261.  No counterpart
+ * in the reference (one Taylor expansion per step). */
+int dh_set_steps_per_series(dh_system *sys, int m);
 /* Run each step's Chebyshev term series as one captured hipGraph (on = 1;
  * the default for N <= 2048) or as individual kernel launches (on = 0).  The graph is
  * captured on the first step with given spectral bounds, term count and buffer
