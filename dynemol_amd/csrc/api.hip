@@ -352,6 +352,9 @@ extern "C" long long dh_system_bytes_prec(int n_orb, int n_wp, int precision) {
     if (precision == 3) b += ld * ld * 2;                       // bf16 residual
     if (precision == 1 || precision == 3) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
     if (precision == 3) b += 2LL * ld * 4 * n_wp * 2;                     // xb16[2]
+    // FP64: the accumulators and coefficient rows of dh_propagate_steps'
+    // multi-step series (allocated on first use)
+    if (precision == 0) b += (long long)MULTI_MAX * (ld * 2 * n_wp + KMAX_ALLOC + 2) * 16;
     return b;
 }
 

@@ -63,7 +63,9 @@ int dh_system_create(dh_context *ctx, int n_atoms, const int *first_orb, const i
                      const double *orb_vshift, const int *orb_frag, int n_frag, int n_wp, dh_system **sys);
 int dh_system_destroy(dh_system *sys);
 /* Device bytes a system of this size holds (serving admission), in FP64 and
- * with the FP32 operator copies of a precision mode (dh_set_precision). */
+ * with the FP32 operator copies of a precision mode (dh_set_precision);
+ * dh_system_bytes_prec(n, w, 0) also counts the accumulators of the
+ * multi-step series of dh_propagate_steps. */
 long long dh_system_bytes(int n_orb, int n_wp);
 long long dh_system_bytes_prec(int n_orb, int n_wp, int precision);
 
