@@ -1489,6 +1489,8 @@ static int multi_series(dh_system* s, double dt, int m, int* kmax) {
     s->coefm_pending = true;
     SeriesOps o{};
     o.H = s->Hth;
+    o.H32 = s->H32;
+    o.Hlo = s->Hlo;
     o.ld = s->ld;
     o.N = s->N;
     o.R = R;
@@ -1497,8 +1499,12 @@ static int multi_series(dh_system* s, double dt, int m, int* kmax) {
     o.x1 = s->x1;
     o.partial = s->partial;
     o.counters = s->counters;
+    o.x32[0] = s->x32[0];
+    o.x32[1] = s->x32[1];
+    o.xb16[0] = s->xb16[0];
+    o.xb16[1] = s->xb16[1];
     o.lib = s->ctx->lib;
-    o.prec = 0;
+    o.prec = s->precision;
     o.c = c;
     o.r = r;
     o.m = m;
@@ -1583,10 +1589,13 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                 DH_HIP(hipEventCreate(&e));
                 s->evpool.push_back(e);
             }
-        // FP64 steps through a kernel whose epilogue takes several accumulators
-        // run as multi-step series (up to MULTI_MAX steps per recurrence)
-        const bool multi_ok = s->precision == 0 && s->steps_per_series != 1 &&
-                              matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL));
+        // steps through a term whose epilogue takes several accumulators run as
+        // multi-step series (up to MULTI_MAX steps per recurrence): FP64 with
+        // the split-K GEMM or a symmetric kernel, and the precision modes (the
+        // library GEMMs' reduction, the FP32-stored operator's symmetric kernel
+        // or split-K GEMM)
+        const bool multi_ok = s->steps_per_series != 1 &&
+                              (s->precision != 0 || matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL)));
         int n_timed = 0, chunk_terms = 0;
         long terms_timed = 0, launches = 0;
         for (int i = j; i < jend;) {
@@ -1606,8 +1615,13 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                 cplx* keep = s->phi;
                 for (int q = 0; q < mc; ++q) {
                     cplx* st_q = s->accm + (size_t)q * acc_n;
-                    launch_column_norms(s->N, R, q == 0 ? keep : st_q - acc_n, d.n0 + (size_t)(i + q) * R, st);
+                    // FP32 products: every step's state takes back the norm of the
+                    // series' input (as the single steps restore each step's), so
+                    // the drift checked is that of q + 1 steps
+                    const bool f32p = fp32_products(s);
+                    launch_column_norms(s->N, R, (q == 0 || f32p) ? keep : st_q - acc_n, d.n0 + (size_t)(i + q) * R, st);
                     launch_column_norms(s->N, R, st_q, d.n1 + (size_t)(i + q) * R, st);
+                    if (f32p) launch_renorm_columns(st_q, s->N, R, d.n0 + (size_t)(i + q) * R, d.n1 + (size_t)(i + q) * R, st);
                     s->phi = st_q;  // the outputs of step i + q
                     launch_outputs(s);
                     s->phi = keep;
