@@ -1594,8 +1594,14 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
         // the split-K GEMM or a symmetric kernel, and the precision modes (the
         // library GEMMs' reduction, the FP32-stored operator's symmetric kernel
         // or split-K GEMM)
+        // The FP32-product modes (1, 3) read 6 bytes or fewer of operator per
+        // element against the accumulators' 32 N R bytes per term per active
+        // step (about 5 of 10 on average): below R = N / 50 the terms saved
+        // outweigh that traffic (N = 8192, R = 128: +20 %; N = 4096, R = 128:
+        // -3 to +1 %)
         const bool multi_ok = s->steps_per_series != 1 &&
-                              (s->precision != 0 || matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL)));
+                              (s->precision == 0 ? matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL))
+                                                 : !fp32_products(s) || 50L * R < s->N);
         int n_timed = 0, chunk_terms = 0;
         long terms_timed = 0, launches = 0;
         for (int i = j; i < jend;) {

@@ -197,7 +197,10 @@ int dh_get_stats(dh_system *sys, double *out9);
 int dh_set_timing(dh_system *sys, int on);
 /* Steps per Chebyshev recurrence in dh_propagate_steps (FP64 systems whose
  * autotuned term kernel is the split-K GEMM or a symmetric half-storage
- * kernel): within a rebuild interval the operator is fixed and the terms
+ * kernel, the FP32-stored operator mode, and the FP32-product modes when the
+ * right-hand sides R = 2 n_wp stay below N / 50, where the accumulators'
+ * traffic stays small against the operator's): within a rebuild interval the
+ * operator is fixed and the terms
  * T_k(Hh) phi do not depend on time, so one recurrence to the last step's K
  * collects every step's state, each with its own coefficients.  m = 0
  * (default): up to 10 steps per recurrence, within the interval and the
