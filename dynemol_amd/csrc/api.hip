@@ -115,6 +115,7 @@ struct dh_system {
     // coefficient rows (KMAX_ALLOC + 2 each; device, and pinned staging)
     int steps_per_series = 0;       // 0: as many as the interval allows; 1: one series per step
     cplx *accm = nullptr, *coefm = nullptr, *coefm_h = nullptr;
+    int *kq_d = nullptr, *kq_h = nullptr;  // the accumulators' last terms (device; pinned staging)
     hipEvent_t coefm_ev = nullptr;
     bool coefm_pending = false;
     // observables
@@ -286,8 +287,10 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->coef_h) hipHostFree(s->coef_h);
     if (s->coefm_ev) hipEventDestroy(s->coefm_ev);
     if (s->coefm_h) hipHostFree(s->coefm_h);
+    if (s->kq_h) hipHostFree(s->kq_h);
     if (s->accm) hipFree(s->accm);
     if (s->coefm) hipFree(s->coefm);
+    if (s->kq_d) hipFree(s->kq_d);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
     if (s->ctx->side) hipStreamSynchronize(s->ctx->side);  // a launched build may still run
     void* shp[] = {s->sh.Q1, s->sh.Q2, s->sh.Hth, s->sh.coords, s->sh.H32, s->sh.Hlo, s->sh.lv, s->sh.lvp, s->sh.lw,
@@ -1445,6 +1448,49 @@ static int multi_chunk_steps(const dh_system* s, double dt, int left) {
     return std::max(m, 1);
 }
 
+// The coefficient rows of an m-step series of s (host recurrence, one copy on
+// the stream): accumulator q's row in s->coefm, its last term Kq[q] (made
+// non-decreasing in q, a shorter row padded with zero coefficients: adding
+// 0 T_k changes nothing) in Kq and s->kq_d.  The buffers are allocated on
+// first use.
+static int multi_coefficients(dh_system* s, double dt, int m, int* Kq) {
+    hipStream_t st = s->ctx->stream;
+    const long acc_n = (long)s->ld * 2 * s->W;
+    constexpr int CS = KMAX_ALLOC + 2;
+    if (!s->accm) {
+        DALLOC(s->accm, (size_t)MULTI_MAX * acc_n);
+        DALLOC(s->coefm, (size_t)MULTI_MAX * CS);
+        DALLOC(s->kq_d, (size_t)MULTI_MAX);
+        DH_HIP(hipHostMalloc((void**)&s->coefm_h, sizeof(cplx) * MULTI_MAX * CS, hipHostMallocDefault));
+        DH_HIP(hipHostMalloc((void**)&s->kq_h, sizeof(int) * MULTI_MAX, hipHostMallocDefault));
+        DH_HIP(hipEventCreateWithFlags(&s->coefm_ev, hipEventDisableTiming));
+    }
+    if (s->coefm_pending) DH_HIP(hipEventSynchronize(s->coefm_ev));  // the staging buffers are free again
+    const double c = 0.5 * (s->emax + s->emin);
+    const double r = 0.5 * (s->emax - s->emin) * s->widen;
+    for (int q = 0; q < m; ++q) {
+        cplx* row = s->coefm_h + (size_t)q * CS;
+        const int k = bessel_coef_host(r * (q + 1) * dt, c * (q + 1) * dt, KMAX_ALLOC, BESSEL_TOL, row);
+        if (k <= 0 && r * dt > 1e-12) {
+            dh::set_error("Chebyshev coefficients: no terms for r*dt = %g", r * (q + 1) * dt);
+            return DH_ERR_STATE;
+        }
+        Kq[q] = k;
+    }
+    for (int q = 1; q < m; ++q)
+        if (Kq[q] < Kq[q - 1]) {
+            cplx* row = s->coefm_h + (size_t)q * CS;
+            for (int k = Kq[q] + 1; k <= Kq[q - 1] + 1; ++k) row[k] = cplx{0.0, 0.0};
+            Kq[q] = Kq[q - 1];
+        }
+    for (int q = 0; q < m; ++q) s->kq_h[q] = Kq[q];
+    DH_HIP(hipMemcpyAsync(s->coefm, s->coefm_h, sizeof(cplx) * (size_t)m * CS, hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpyAsync(s->kq_d, s->kq_h, sizeof(int) * m, hipMemcpyHostToDevice, st));
+    DH_HIP(hipEventRecord(s->coefm_ev, st));
+    s->coefm_pending = true;
+    return DH_OK;
+}
+
 // m steps of dt from s->phi in one Chebyshev recurrence: the terms T_k(Hh) phi
 // do not depend on time, so step q's state sum_k a_k((q + 1) dt) T_k phi
 // collects in accumulator q of s->accm with its own coefficients, all from the
@@ -1457,36 +1503,10 @@ static int multi_series(dh_system* s, double dt, int m, int* kmax) {
     const int R = 2 * s->W;
     const long acc_n = (long)s->ld * R;
     constexpr int CS = KMAX_ALLOC + 2;
-    if (!s->accm) {
-        DALLOC(s->accm, (size_t)MULTI_MAX * acc_n);
-        DALLOC(s->coefm, (size_t)MULTI_MAX * CS);
-        DH_HIP(hipHostMalloc((void**)&s->coefm_h, sizeof(cplx) * MULTI_MAX * CS, hipHostMallocDefault));
-        DH_HIP(hipEventCreateWithFlags(&s->coefm_ev, hipEventDisableTiming));
-    }
-    if (s->coefm_pending) DH_HIP(hipEventSynchronize(s->coefm_ev));  // coefm_h is free again
+    int Kq[MULTI_MAX];
+    if (int rc = multi_coefficients(s, dt, m, Kq)) return rc;
     const double c = 0.5 * (s->emax + s->emin);
     const double r = 0.5 * (s->emax - s->emin) * s->widen;
-    int Kq[MULTI_MAX];
-    for (int q = 0; q < m; ++q) {
-        cplx* row = s->coefm_h + (size_t)q * CS;
-        const int k = bessel_coef_host(r * (q + 1) * dt, c * (q + 1) * dt, KMAX_ALLOC, BESSEL_TOL, row);
-        if (k <= 0 && r * dt > 1e-12) {
-            dh::set_error("Chebyshev coefficients: no terms for r*dt = %g", r * (q + 1) * dt);
-            return DH_ERR_STATE;
-        }
-        Kq[q] = k;
-    }
-    // the accumulators leave the series in order: K_q non-decreasing in q, a
-    // shorter row padded with zero coefficients (adding 0 T_k changes nothing)
-    for (int q = 1; q < m; ++q)
-        if (Kq[q] < Kq[q - 1]) {
-            cplx* row = s->coefm_h + (size_t)q * CS;
-            for (int k = Kq[q] + 1; k <= Kq[q - 1] + 1; ++k) row[k] = cplx{0.0, 0.0};
-            Kq[q] = Kq[q - 1];
-        }
-    DH_HIP(hipMemcpyAsync(s->coefm, s->coefm_h, sizeof(cplx) * (size_t)m * CS, hipMemcpyHostToDevice, st));
-    DH_HIP(hipEventRecord(s->coefm_ev, st));
-    s->coefm_pending = true;
     SeriesOps o{};
     o.H = s->Hth;
     o.H32 = s->H32;
@@ -2275,11 +2295,11 @@ static int g_batch_family[6] = {-1, -1, -1, -1, -1, -1};
 static std::mutex g_batch_family_mu;
 static int batch_family_slot(int R) { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : R == 32 ? 5 : 0; }
 
-static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k, hipStream_t st) {
+static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k, hipStream_t st, bool multi = false) {
     const int n = b->n;
     if (sym)
         launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
-                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, b->reduce_groups, st);
+                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, b->reduce_groups, st, multi);
     else
         launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], b->R, k,
                               st);
@@ -2287,19 +2307,20 @@ static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k
 
 // Chebyshev terms of a batch's grouped members: those with K_s >= k are the
 // first `active` of `order`
-static void batch_series_launches(dh_batch* b, hipStream_t st) {
+static void batch_series_launches(dh_batch* b, hipStream_t st, bool multi = false) {
     const int n = b->n;
     int active = n;
     for (int k = 1; k <= b->Kmax; ++k) {
         while (active > 0 && b->Ks[b->order[active - 1]] < k) --active;
         const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
-        batch_term_launch(b, b->sym, kind, active, k, st);
+        batch_term_launch(b, b->sym, kind, active, k, st, multi);
     }
 }
 
-static std::vector<double> batch_graph_key(const dh_batch* b) {
+static std::vector<double> batch_graph_key(const dh_batch* b, bool multi) {
     std::vector<double> key(b->Ks.begin(), b->Ks.end());
     key.push_back(b->sym ? 1.0 : 0.0);
+    key.push_back(multi ? 1.0 : 0.0);
     for (int i = 0; i < b->n; ++i) key.push_back(b->solo[i] ? 1.0 : 0.0);
     return key;
 }
@@ -2313,19 +2334,19 @@ static int batch_solo_series(dh_batch* b) {
     return DH_OK;
 }
 
-static int batch_series(dh_batch* b, hipStream_t st) {
+static int batch_series(dh_batch* b, hipStream_t st, bool multi = false) {
     if (!b->use_graph || b->Kmax == 0) {
-        batch_series_launches(b, st);
+        batch_series_launches(b, st, multi);
         return batch_solo_series(b);
     }
-    std::vector<double> key = batch_graph_key(b);
+    std::vector<double> key = batch_graph_key(b, multi);
     if (!b->gexec || b->gkey != key) {
         if (b->gexec) {
             DH_HIP(hipGraphExecDestroy(b->gexec));
             b->gexec = nullptr;
         }
         DH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        batch_series_launches(b, st);
+        batch_series_launches(b, st, multi);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(st, &graph);
         if (e != hipSuccess) {
@@ -2527,9 +2548,83 @@ static int batch_autotune(dh_batch* b, hipStream_t st) {
     return DH_OK;
 }
 
+extern "C" int dh_test_force_batch_family(int R, int family) {
+    CHECK_ARG((R == 2 || R == 4 || R == 8 || R == 16 || R == 32) && family >= -1 && family <= 1,
+              "dh_test_force_batch_family: bad argument");
+    std::lock_guard<std::mutex> lk(g_batch_family_mu);
+    g_batch_family[batch_family_slot(R)] = family;
+    return DH_OK;
+}
+
 extern "C" int dh_batch_tuning(dh_batch* b, int* family) {
     CHECK_ARG(b && family, "dh_batch_tuning: null argument");
     *family = b->sym ? 1 : 0;
+    return DH_OK;
+}
+
+// Task tables of a batch for its members' term counts Ks (descending order;
+// members that run their own series add no grouped term tasks)
+static int batch_retable(dh_batch* b, const std::vector<int>& Ks, const std::vector<char>& solo, hipStream_t st) {
+    const int n = b->n, R = b->R, W = b->W;
+    b->Ks = Ks;
+    b->solo = solo;
+    b->order.resize(n);
+    for (int i = 0; i < n; ++i) b->order[i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return Ks[x] > Ks[y]; });
+    std::vector<int> td, tb;
+    b->prefix.assign(1, 0);
+    const int cpb = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
+    for (int q : b->order) {
+        const int nbq = solo[q] ? 0 : (b->sys[q]->N + cpb - 1) / cpb;
+        for (int blk = 0; blk < nbq; ++blk) {
+            td.push_back(q);
+            tb.push_back(blk);
+        }
+        b->prefix.push_back((int)td.size());
+    }
+    // output tasks: every member, after the term tasks
+    const int nterm = (int)td.size();
+    for (int q = 0; q < n; ++q)
+        for (int blk = 0; blk < (b->sys[q]->N + cpb - 1) / cpb; ++blk) {
+            td.push_back(q);
+            tb.push_back(blk);
+        }
+    b->prefix.push_back(nterm);  // prefix[n + 1]: start of the output tasks
+    b->prefix.push_back((int)td.size());
+    DH_HIP(hipMemcpyAsync(b->d_task_desc, td.data(), sizeof(int) * td.size(), hipMemcpyHostToDevice, st));
+    DH_HIP(hipMemcpyAsync(b->d_task_blk, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
+    if (b->sym_ok) {  // the same member order for the symmetric term's two phases
+        std::vector<int> d1, i1, d2, i2;
+        b->prefix1.assign(1, 0);
+        b->prefix2.assign(1, 0);
+        for (int q : b->order) {
+            if (!solo[q]) {
+                int S = 0;
+                const int nseg = sym_term_segments(b->sys[q]->ld, &S);
+                for (int g = 0; g < nseg; ++g) {
+                    d1.push_back(q);
+                    i1.push_back(g);
+                }
+                const int nbq = b->sys[q]->ld / TILE;
+                for (int v = 0; v < nbq * R; ++v) {
+                    d2.push_back(q);
+                    i2.push_back(v);
+                }
+            }
+            b->prefix1.push_back((int)d1.size());
+            b->prefix2.push_back((int)d2.size());
+        }
+        if (!d1.empty()) {
+            DH_HIP(hipMemcpyAsync(b->d_t1_desc, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t1_idx, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
+        }
+    }
+    b->Kmax = 0;
+    for (int i = 0; i < n; ++i)
+        if (!solo[i]) b->Kmax = std::max(b->Kmax, Ks[i]);
+    DH_HIP(hipStreamSynchronize(st));  // the task vectors above are pageable and local
     return DH_OK;
 }
 
@@ -2562,67 +2657,8 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     std::vector<int> Ks(n);
     for (int i = 0; i < n; ++i) Ks[i] = b->sys[i]->K;
     const bool retable = Ks != b->Ks || solo != b->solo || b->prefix.empty();
-    if (retable) {
-        b->Ks = Ks;
-        b->solo = solo;
-        b->order.resize(n);
-        for (int i = 0; i < n; ++i) b->order[i] = i;
-        std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return Ks[x] > Ks[y]; });
-        std::vector<int> td, tb;
-        b->prefix.assign(1, 0);
-        const int cpb = std::min(grouped_cols_per_block(R), grouped_cols_per_block(W));
-        for (int q : b->order) {
-            const int nbq = solo[q] ? 0 : (b->sys[q]->N + cpb - 1) / cpb;
-            for (int blk = 0; blk < nbq; ++blk) {
-                td.push_back(q);
-                tb.push_back(blk);
-            }
-            b->prefix.push_back((int)td.size());
-        }
-        // output tasks: every member, after the term tasks
-        const int nterm = (int)td.size();
-        for (int q = 0; q < n; ++q)
-            for (int blk = 0; blk < (b->sys[q]->N + cpb - 1) / cpb; ++blk) {
-                td.push_back(q);
-                tb.push_back(blk);
-            }
-        b->prefix.push_back(nterm);  // prefix[n + 1]: start of the output tasks
-        b->prefix.push_back((int)td.size());
-        DH_HIP(hipMemcpyAsync(b->d_task_desc, td.data(), sizeof(int) * td.size(), hipMemcpyHostToDevice, st));
-        DH_HIP(hipMemcpyAsync(b->d_task_blk, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
-        if (b->sym_ok) {  // the same member order for the symmetric term's two phases
-            std::vector<int> d1, i1, d2, i2;
-            b->prefix1.assign(1, 0);
-            b->prefix2.assign(1, 0);
-            for (int q : b->order) {
-                if (!solo[q]) {
-                    int S = 0;
-                    const int nseg = sym_term_segments(b->sys[q]->ld, &S);
-                    for (int g = 0; g < nseg; ++g) {
-                        d1.push_back(q);
-                        i1.push_back(g);
-                    }
-                    const int nbq = b->sys[q]->ld / TILE;
-                    for (int v = 0; v < nbq * R; ++v) {
-                        d2.push_back(q);
-                        i2.push_back(v);
-                    }
-                }
-                b->prefix1.push_back((int)d1.size());
-                b->prefix2.push_back((int)d2.size());
-            }
-            if (!d1.empty()) {
-                DH_HIP(hipMemcpyAsync(b->d_t1_desc, d1.data(), sizeof(int) * d1.size(), hipMemcpyHostToDevice, st));
-                DH_HIP(hipMemcpyAsync(b->d_t1_idx, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice, st));
-                DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
-                DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
-            }
-        }
-        b->Kmax = 0;
-        for (int i = 0; i < n; ++i)
-            if (!solo[i]) b->Kmax = std::max(b->Kmax, Ks[i]);
-        DH_HIP(hipStreamSynchronize(st));  // the task vectors above are pageable and local
-    }
+    if (retable)
+        if (int rc = batch_retable(b, Ks, solo, st)) return rc;
     // per-member descriptors, norm tasks: uploaded when they change (bounds,
     // buffers); pointers are read on the device every step
     std::vector<MatvecArgs> args((size_t)dh_batch::NDESC * n);
@@ -2704,6 +2740,166 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
     b->stats[1] += b->Kmax;
     b->stats[2] = maxdrift;
     if (max_terms) *max_terms = *std::max_element(Ks.begin(), Ks.end()) + 1;
+    return DH_OK;
+}
+
+// n_steps steps of dt_ps for every member, up to MULTI_MAX steps per
+// recurrence (the multi-step series of dh_propagate_steps, grouped): each
+// member's accumulators take their own coefficient rows, and the grouped
+// phase-2 kernel finds a member's first active accumulator from its term table.
+// One norm check, commit and output pass per recurrence; the outputs are those
+// of the final state.  Batches whose members are not all FP64, or whose term
+// family is the dense one, step one step at a time (dh_batch_propagate).
+extern "C" int dh_batch_propagate_steps(dh_batch* b, int n_steps, double dt_ps, int* max_terms) {
+    CHECK_ARG(b && n_steps > 0 && dt_ps > 0.0, "dh_batch_propagate_steps: need a batch, n_steps > 0 and dt > 0");
+    DH_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const double dt = dt_ps / H_BAR;
+    const int n = b->n, R = b->R;
+    int done = 0, kmax_seen = 0;
+    bool all_fp64 = true;
+    for (auto* s : b->sys) all_fp64 = all_fp64 && s->precision == 0;
+    // the first step tunes the batch (kernel family, task tables) as usual
+    if (b->prefix.empty() || !all_fp64 || !b->sym_ok) {
+        int k = 0;
+        if (int rc = dh_batch_propagate(b, 0.0, dt_ps, &k)) return rc;
+        kmax_seen = k;
+        done = 1;
+    }
+    if (!all_fp64 || !b->sym) {
+        for (; done < n_steps; ++done) {
+            int k = 0;
+            if (int rc = dh_batch_propagate(b, 0.0, dt_ps, &k)) return rc;
+            kmax_seen = std::max(kmax_seen, k);
+        }
+        if (max_terms) *max_terms = kmax_seen;
+        return DH_OK;
+    }
+    const long cs = KMAX_ALLOC + 2;
+    while (done < n_steps) {
+        int m = std::min(MULTI_MAX, n_steps - done);
+        for (auto* s : b->sys) {
+            if (s->phi_dirty) {
+                convert_ao_to_phi(s);
+                s->phi_dirty = false;
+            }
+            m = std::min(m, multi_chunk_steps(s, dt, m));
+        }
+        if (m < 2) {  // a member's coefficient table holds one step only
+            int k = 0;
+            if (int rc = dh_batch_propagate(b, 0.0, dt_ps, &k)) return rc;
+            kmax_seen = std::max(kmax_seen, k);
+            ++done;
+            continue;
+        }
+        std::vector<int> Ks(n);
+        for (int i = 0; i < n; ++i) {
+            int Kq[MULTI_MAX];
+            if (int rc = multi_coefficients(b->sys[i], dt, m, Kq)) return rc;
+            Ks[i] = Kq[m - 1];
+        }
+        std::vector<char> solo(n, 0);
+        if (Ks != b->Ks || solo != b->solo || b->prefix.empty())
+            if (int rc = batch_retable(b, Ks, solo, st)) return rc;
+        // term descriptors with the members' accumulators; the output
+        // descriptors read phi, which the commit below fills
+        std::vector<MatvecArgs> args((size_t)dh_batch::NDESC * n);
+        for (int kind = 0; kind < dh_batch::NDESC; ++kind)
+            for (int i = 0; i < n; ++i) {
+                dh_system* s = b->sys[i];
+                MatvecArgs a = batch_desc(s, kind);
+                if (kind <= dh_batch::D_KEVEN) {
+                    const double c = 0.5 * (s->emax + s->emin), r = 0.5 * (s->emax - s->emin) * s->widen;
+                    const double f = kind == dh_batch::D_K1 ? 1.0 : 2.0;
+                    a.alpha = f / r;
+                    a.beta = -f * c / r;
+                    a.acc = nullptr;
+                    a.coef = nullptr;
+                    a.accm = s->accm;
+                    a.accm_stride = (long)s->ld * R;
+                    a.coefm = s->coefm;
+                    a.coefm_stride = (int)cs;
+                    a.m_lo = 0;
+                    a.m_hi = m;
+                    a.kq = s->kq_d;
+                }
+                args[(size_t)kind * n + i] = a;
+            }
+        std::vector<NormTask> nt((size_t)2 * n * R);
+        for (int i = 0; i < n; ++i) {
+            dh_system* s = b->sys[i];
+            const cplx* last = s->accm + (size_t)(m - 1) * s->ld * R;
+            for (int c = 0; c < R; ++c) {
+                nt[(size_t)i * R + c] = NormTask{s->phi, s->N, R, c, b->d_nval + (size_t)i * R + c};
+                nt[(size_t)n * R + (size_t)i * R + c] = NormTask{last, s->N, R, c, b->d_nval + (size_t)n * R + (size_t)i * R + c};
+            }
+        }
+        if (int rc = upload_if_changed(b->h_args, args, b->d_args, st)) return rc;
+        if (int rc = upload_if_changed(b->h_norms, nt, b->d_norms, st)) return rc;
+        launch_grouped_norms(b->d_norms, n * R, st);
+        if (int rc = batch_series(b, st, true)) return rc;
+        launch_grouped_norms(b->d_norms + (size_t)n * R, n * R, st);
+        std::vector<double> nv((size_t)2 * n * R);
+        DH_HIP(hipMemcpyAsync(nv.data(), b->d_nval, sizeof(double) * nv.size(), hipMemcpyDeviceToHost, st));
+        DH_HIP(hipStreamSynchronize(st));
+        std::vector<int> ok(n, 1);
+        double maxdrift = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < R; ++c) {
+                const double n0 = nv[(size_t)i * R + c], n1 = nv[(size_t)n * R + (size_t)i * R + c];
+                const double d = n0 > 0 ? std::fabs(n1 - n0) / n0 : 0.0;
+                if (!(d < 1e-9)) ok[i] = 0;
+                if (std::isfinite(d)) maxdrift = std::max(maxdrift, d);
+            }
+        // commit the last step's state for members whose norm held
+        std::vector<CopyTask> ct;
+        for (int i = 0; i < n; ++i)
+            if (ok[i]) {
+                dh_system* s = b->sys[i];
+                ct.push_back(CopyTask{s->accm + (size_t)(m - 1) * s->ld * R, s->phi, (long)s->N * R});
+            }
+        if (int rc = upload_if_changed(b->h_copy, ct, b->d_copy, st)) return rc;
+        launch_grouped_copy(b->d_copy, (int)ct.size(), st);
+        const int W = b->W;
+        const int o0 = b->prefix[n + 1], o1 = b->prefix[n + 2];
+        launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_ENERGY * n, b->d_task_desc + o0, b->d_task_blk + o0,
+                              o1 - o0, W, 0, st);
+        launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LBRA * n, b->d_task_desc + o0, b->d_task_blk + o0,
+                              o1 - o0, W, 0, st);
+        launch_matvec_grouped(b->d_args + (size_t)dh_batch::D_LINVT * n, b->d_task_desc + o0, b->d_task_blk + o0,
+                              o1 - o0, R, 0, st);
+        std::vector<ObsTask> ot((size_t)n * W);
+        for (int i = 0; i < n; ++i) {
+            dh_system* s = b->sys[i];
+            for (int w = 0; w < W; ++w)
+                ot[(size_t)i * W + w] =
+                    ObsTask{s->N, W, s->n_frag, w, s->frag, s->out, s->phi, s->hk, s->pops, s->erg, s->norms};
+        }
+        if (int rc = upload_if_changed(b->h_obs, ot, b->d_obs, st)) return rc;
+        launch_grouped_observables(b->d_obs, n * W, st);
+        DH_HIP(hipGetLastError());
+        DH_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < n; ++i) {
+            dh_system* s = b->sys[i];
+            if (ok[i]) {
+                s->stats[0] = Ks[i] + 1;
+                s->stats[1] += m;
+                s->stats[2] += Ks[i] + 3;
+                continue;
+            }
+            // bounds did not enclose this member's spectrum: its m steps alone,
+            // with the checked step loop (wider bounds where needed)
+            s->stats[3] += 1;
+            b->stats[3] += 1;
+            if (int rc = dh_propagate_steps(s, m, dt_ps, 0, nullptr, 0.0, 0.0, nullptr, nullptr, nullptr)) return rc;
+        }
+        b->stats[0] += m;
+        b->stats[1] += b->Kmax;
+        b->stats[2] = maxdrift;
+        kmax_seen = std::max(kmax_seen, b->Kmax + 1);
+        done += m;
+    }
+    if (max_terms) *max_terms = kmax_seen;
     return DH_OK;
 }
 

@@ -842,11 +842,21 @@ struct MultiSpread {
     cplx am[QP], cq[QP];
 };
 
+// the first accumulator taking term a.k: a.m_lo, or for a batch member (a.kq)
+// the first whose series reaches k (the last one always does)
+__device__ __forceinline__ int multi_first(const MatvecArgs& a) {
+    if (!a.kq) return a.m_lo;
+    int q = 0;
+    while (q < a.m_hi - 1 && a.kq[q] < a.k) ++q;
+    return q;
+}
+
 template <int G>
-__device__ __forceinline__ void multi_spread_load(const MatvecArgs& a, int g, long oi, bool valid, MultiSpread<G>& ms) {
+__device__ __forceinline__ void multi_spread_load(const MatvecArgs& a, int m_lo, int g, long oi, bool valid,
+                                                  MultiSpread<G>& ms) {
 #pragma unroll
     for (int v = 0; v < MultiSpread<G>::QP; ++v) {
-        const int q = a.m_lo + g + v * G;
+        const int q = m_lo + g + v * G;
         if (valid && q < a.m_hi) {
             ms.am[v] = a.accm[(long)q * a.accm_stride + oi];
             ms.cq[v] = a.coefm[(long)q * a.coefm_stride + a.k];
@@ -855,11 +865,11 @@ __device__ __forceinline__ void multi_spread_load(const MatvecArgs& a, int g, lo
 }
 
 template <int G>
-__device__ __forceinline__ void multi_spread_store(const MatvecArgs& a, int g, long oi, bool valid,
+__device__ __forceinline__ void multi_spread_store(const MatvecArgs& a, int m_lo, int g, long oi, bool valid,
                                                    const MultiSpread<G>& ms, cplx T) {
 #pragma unroll
     for (int v = 0; v < MultiSpread<G>::QP; ++v) {
-        const int q = a.m_lo + g + v * G;
+        const int q = m_lo + g + v * G;
         if (valid && q < a.m_hi) {
             const cplx add = cmul(ms.cq[v], T);
             a.accm[(long)q * a.accm_stride + oi] = cplx{ms.am[v].re + add.re, ms.am[v].im + add.im};
@@ -1399,7 +1409,8 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
     MultiSpread<G> ms;
     const bool spread = MULTI && a.epi == EPI_CHEB;
-    if (spread) multi_spread_load<G>(a, g, oi, i < a.N, ms);
+    const int m_lo = MULTI ? multi_first(a) : 0;
+    if (spread) multi_spread_load<G>(a, m_lo, g, oi, i < a.N, ms);
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -1437,7 +1448,7 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     }
     if (spread) {
         __syncthreads();  // T_k of the 64 elements in LDS
-        multi_spread_store<G>(a, g, oi, i < a.N, ms, Ts[t & 63]);
+        multi_spread_store<G>(a, m_lo, g, oi, i < a.N, ms, Ts[t & 63]);
     }
 }
 
@@ -1461,7 +1472,7 @@ static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st
 
 // Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),
 // with the system's segment length sys_S[task_desc[i]]
-template <int R, int G>
+template <int R, int G, bool MULTI = false>
 __global__ __launch_bounds__(G * 64) void sym_reduce_grouped_kernel(const MatvecArgs* __restrict__ args,
                                                                     const int* __restrict__ task_desc,
                                                                     const int* __restrict__ task_idx,
@@ -1469,7 +1480,7 @@ __global__ __launch_bounds__(G * 64) void sym_reduce_grouped_kernel(const Matvec
     const int d = task_desc[blockIdx.x], v = task_idx[blockIdx.x];
     MatvecArgs a = args[d];
     a.k = k;
-    sym_reduce_body<R, G>(a, sys_S[d], v / R, v % R);
+    sym_reduce_body<R, G, MULTI>(a, sys_S[d], v / R, v % R);
 }
 
 template <int R>
@@ -1672,7 +1683,8 @@ int sym_term_segments(int ld, int* S) {
 }
 
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st) {
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st,
+                        bool multi) {
     if (n1 > 0) switch (R) {
             case 1: hipLaunchKernelGGL(symm_tiles_grouped_kernel<1>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
             case 2: hipLaunchKernelGGL(symm_tiles_grouped_kernel<2>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
@@ -1681,9 +1693,15 @@ void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int*
             default: break;
         }
     if (n2 <= 0) return;
-#define DH_SYM_RED(RR, GG)                                                                                      \
-    hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG>), dim3(n2), dim3(GG * 64), 0, st, d_args, t2_desc, t2_idx, \
-                       sys_S, k)
+#define DH_SYM_RED(RR, GG)                                                                                             \
+    do {                                                                                                               \
+        if (multi)                                                                                                     \
+            hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG, true>), dim3(n2), dim3(GG * 64), 0, st, d_args,      \
+                               t2_desc, t2_idx, sys_S, k);                                                             \
+        else                                                                                                           \
+            hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG, false>), dim3(n2), dim3(GG * 64), 0, st, d_args,     \
+                               t2_desc, t2_idx, sys_S, k);                                                             \
+    } while (0)
     if (reduce_groups == 16) switch (R) {
             case 1: DH_SYM_RED(1, 16); break;
             case 2: DH_SYM_RED(2, 16); break;
@@ -2006,7 +2024,8 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
     if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
     MultiSpread<SYM_G> ms;
     const bool spread = MULTI && a.epi == EPI_CHEB;
-    if (spread) multi_spread_load<SYM_G>(a, g, oi, i < a.N, ms);
+    const int m_lo = MULTI ? multi_first(a) : 0;
+    if (spread) multi_spread_load<SYM_G>(a, m_lo, g, oi, i < a.N, ms);
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -2044,7 +2063,7 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
     }
     if (spread) {
         __syncthreads();  // T_k of the 64 elements in LDS
-        multi_spread_store<SYM_G>(a, g, oi, i < a.N, ms, Ts[t & 63]);
+        multi_spread_store<SYM_G>(a, m_lo, g, oi, i < a.N, ms, Ts[t & 63]);
     }
 }
 

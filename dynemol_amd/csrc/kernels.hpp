@@ -154,6 +154,8 @@ struct MatvecArgs {
     const cplx* coefm;
     int coefm_stride;
     int m_lo, m_hi;
+    const int* kq;  // batch members: the accumulators' last terms (device), from which
+                    // the kernel takes m_lo = first q < m_hi - 1 with kq[q] >= k
 };
 constexpr int MULTI_MAX = 10;  // accumulators of one multi-step series
 bool matvec_variant_multi_ok(int variant);
@@ -190,7 +192,7 @@ int sym_term_segments(int ld, int* S);
 // reduce_groups: the phase-2 group count of the batch's systems (sym_reduce_groups_for
 // their ld; a batch whose systems differ in it uses the largest)
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st);
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st, bool multi = false);
 int sym_reduce_groups_for(int ld);
 
 void launch_zero_u32(unsigned* p, int n, hipStream_t st);

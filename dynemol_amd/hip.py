@@ -34,14 +34,14 @@ SYMBOLS = [
     "dh_system_bytes", "dh_system_bytes_prec", "dh_set_coords", "dh_build_operators", "dh_load_operators", "dh_orthogonalize",
     "dh_get_matrix", "dh_get_bounds", "dh_set_lanczos", "dh_set_wavepacket", "dh_get_wavepacket", "dh_propagate",
     "dh_get_observables", "dh_get_stats", "dh_set_timing", "dh_set_steps_per_series", "dh_matvec_tuning", "dh_set_precision",
-    "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_set_initial_state",
+    "dh_propagation_elhl", "dh_batch_create", "dh_batch_destroy", "dh_batch_propagate", "dh_batch_propagate_steps", "dh_set_initial_state",
     "dh_set_graph", "dh_ehrenfest_kernel", "dh_eigenpairs", "dh_set_eigenstate", "dh_batch_set_graph",
     "dh_chebyshev_work_bytes", "dh_chebyshev_apply", "dh_chebyshev_apply_noalloc", "dh_propagate_steps",
     "dh_batch_tuning",
 ]
 TEST_SYMBOLS = ["dh_test_gemm", "dh_test_cholesky", "dh_test_copy", "dh_test_force_matvec_variant",
                 "dh_test_matvec_timing", "dh_test_read", "dh_test_device_alloc", "dh_test_device_free",
-                "dh_test_memcpy", "dh_test_bessel_coefficients"]
+                "dh_test_memcpy", "dh_test_bessel_coefficients", "dh_test_force_batch_family"]
 
 # Every live context and system is released before interpreter shutdown, while
 # the HIP runtime is still up (a handle freed during finalisation aborts).
@@ -115,6 +115,7 @@ def lib():
     L.dh_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.dh_batch_destroy.argtypes = [_vp]
     L.dh_batch_propagate.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _i]
+    L.dh_batch_propagate_steps.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _i]
     L.dh_batch_tuning.argtypes = [_vp, _i]
     L.dh_test_force_matvec_variant.argtypes = [ctypes.c_int]
     L.dh_test_matvec_timing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
@@ -408,6 +409,14 @@ class Batch:
         _check(lib().dh_batch_propagate(self.h, t_init, t_max, ctypes.byref(n)), "dh_batch_propagate")
         return n.value
 
+    def propagate_steps(self, n_steps, dt):
+        """n_steps steps of dt (ps) for every member, several steps per recurrence;
+        returns the most terms of one recurrence."""
+        n = ctypes.c_int(0)
+        _check(lib().dh_batch_propagate_steps(self.h, int(n_steps), float(dt), ctypes.byref(n)),
+               "dh_batch_propagate_steps")
+        return n.value
+
     def set_graph(self, on=True):
         """Term series as one captured hipGraph per step (default) or as separate launches."""
         _check(lib().dh_batch_set_graph(self.h, 1 if on else 0), "dh_batch_set_graph")
@@ -422,6 +431,10 @@ class Batch:
         if self.h:
             lib().dh_batch_destroy(self.h)
             self.h = _vp()
+
+
+def force_batch_family(R, family):
+    _check(lib().dh_test_force_batch_family(int(R), int(family)), "dh_test_force_batch_family")
 
 
 def force_matvec_variant(v):

@@ -234,6 +234,16 @@ typedef struct dh_batch dh_batch;
 int dh_batch_create(dh_context *ctx, int n, dh_system *const *systems, dh_batch **batch);
 int dh_batch_destroy(dh_batch *batch);
 int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_terms);
+/* n_steps steps of dt (ps) for every member, up to 10 steps per Chebyshev
+ * recurrence (each member's steps collected from the same terms with their
+ * own coefficients, as dh_propagate_steps does for one system): one norm
+ * check, commit and output pass per recurrence, and the outputs read
+ * afterwards are those of the final state.  The states agree with n_steps
+ * calls of dh_batch_propagate to rounding.  Batches with members in other
+ * precision modes, or whose autotuned term family is the dense one, step one
+ * step at a time.  max_terms: the most terms of one recurrence.  No counterpart
+ * in the reference. */
+int dh_batch_propagate_steps(dh_batch *batch, int n_steps, double dt, int *max_terms);
 /* Term kernel family the batch autotuner chose for this batch's wavepacket
  * count (once per process, by timing both on the first batch of that count):
  * 1 = symmetric half-storage tiles, 0 = dense column blocks.  Members in a

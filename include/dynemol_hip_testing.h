@@ -39,6 +39,10 @@ int dh_test_memcpy(void *dst, const void *src, long long bytes, int to_device);
  * 0 at K + 1, where K is one past the last k with |J_k(x)| > tol (at most the
  * recurrence's start index), and *K_out = K. */
 int dh_test_bessel_coefficients(double x, double phase, double tol, int kmax, double *coef, int *K_out);
+/* Set the batch term family used for R right-hand sides (1 symmetric half
+ * storage, 0 dense column blocks; -1: the autotuner chooses again), for the
+ * batches created afterwards. */
+int dh_test_force_batch_family(int R, int family);
 #ifdef __cplusplus
 }
 #endif

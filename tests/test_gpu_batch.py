@@ -69,6 +69,51 @@ def test_batch_matches_single_and_exact(n_wp):
         batch.close()
 
 
+@pytest.mark.parametrize("family", [1, 0])
+def test_batch_steps_match_single_steps(family):
+    """dh_batch_propagate_steps (10 steps per recurrence for a symmetric-family
+    batch; one at a time for the dense family) against each system stepped
+    alone with dh_propagate, and against the exact propagator: 13 steps (the
+    tuning step, a series of 10, a series of 2)."""
+    sizes = [300, 517, 256, 800, 411, 1000]
+    n_wp, steps = 4, 13
+    _say(f"batch steps of {len(sizes)} systems, family {family}, {steps} steps")
+    data = [_system_inputs(200 + i, n, n_wp) for i, n in enumerate(sizes)]
+    hip.force_batch_family(2 * n_wp, family)
+    try:
+        with hip.Context(0) as ctx:
+            batched, single = [], []
+            for st, b, S, E, C, kets in data:
+                for group in (batched, single):
+                    s = hip.System(ctx, b, n_wp=n_wp, n_frag=2)
+                    s.build(st.coords)
+                    for w, k in enumerate(kets):
+                        s.set_wavepacket(w, S @ k, k)
+                    group.append(s)
+            batch = hip.Batch(ctx, batched)
+            kmax = batch.propagate_steps(steps, DT)
+            assert batch.tuning() == ("symmetric" if family == 1 else "dense")
+            for s in single:
+                for _ in range(steps):
+                    s.propagate(0.0, DT)
+            _say(f"  most terms of one recurrence: {kmax}")
+            for (st, b, S, E, C, kets), sb, ss in zip(data, batched, single):
+                for w in range(n_wp):
+                    bb, kb = sb.wavepacket(w)
+                    bs, ks = ss.wavepacket(w)
+                    assert np.abs(kb - ks).max() <= 1e-11 * np.abs(ks).max()
+                    assert np.abs(bb - bs).max() <= 1e-11 * np.abs(bs).max()
+                    ke = C @ (np.exp(-1j * E * steps * DT / O.H_BAR) * (C.T @ (S @ kets[w])))
+                    assert np.abs(kb - ke).max() <= 1e-9 * np.abs(ke).max()
+                    _, pb, eb = sb.observables(w)
+                    _, ps, es = ss.observables(w)
+                    assert np.abs(pb - ps).max() < 1e-12
+                    assert abs(eb - es) < 1e-10
+            batch.close()
+    finally:
+        hip.force_batch_family(2 * n_wp, -1)
+
+
 def test_batch_graph_replays_match_launches():
     """The captured term series (default) against individual launches, bitwise, over
     steps that replay one graph, with host reads between the replays, and a step of
