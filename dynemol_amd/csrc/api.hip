@@ -112,9 +112,10 @@ struct dh_system {
     hipEvent_t coef_ev = nullptr;   // the last copy out of coef_h
     bool coef_pending = false;
     // multi-step series of dh_propagate_steps: MULTI_MAX accumulators and their
-    // coefficient rows (KMAX_ALLOC + 2 each; device, and pinned staging)
+    // coefficient rows (KMAX_ALLOC + 2 each; device, and pinned staging), and
+    // the PANEL-slot ring of the recurrence's terms
     int steps_per_series = 0;       // 0: as many as the interval allows; 1: one series per step
-    cplx *accm = nullptr, *coefm = nullptr, *coefm_h = nullptr;
+    cplx *accm = nullptr, *coefm = nullptr, *coefm_h = nullptr, *ring = nullptr;
     int *kq_d = nullptr, *kq_h = nullptr;  // the accumulators' last terms (device; pinned staging)
     hipEvent_t coefm_ev = nullptr;
     bool coefm_pending = false;
@@ -289,6 +290,7 @@ extern "C" int dh_system_destroy(dh_system* s) {
     if (s->coefm_h) hipHostFree(s->coefm_h);
     if (s->kq_h) hipHostFree(s->kq_h);
     if (s->accm) hipFree(s->accm);
+    if (s->ring) hipFree(s->ring);
     if (s->coefm) hipFree(s->coefm);
     if (s->kq_d) hipFree(s->kq_d);
     for (hipEvent_t e : s->evpool) hipEventDestroy(e);
@@ -355,9 +357,9 @@ extern "C" long long dh_system_bytes_prec(int n_orb, int n_wp, int precision) {
     if (precision == 3) b += ld * ld * 2;                       // bf16 residual
     if (precision == 1 || precision == 3) b += 2LL * ld * 4 * n_wp * 4;  // x32[2]
     if (precision == 3) b += 2LL * ld * 4 * n_wp * 2;                     // xb16[2]
-    // FP64: the accumulators and coefficient rows of dh_propagate_steps'
+    // the accumulators, coefficient rows and term ring of dh_propagate_steps'
     // multi-step series (allocated on first use)
-    if (precision == 0) b += (long long)MULTI_MAX * (ld * 2 * n_wp + KMAX_ALLOC + 2) * 16;
+    b += ((long long)MULTI_MAX * (ld * 2 * n_wp + KMAX_ALLOC + 2) + PANEL * ld * 2 * n_wp) * 16;
     return b;
 }
 
@@ -1014,14 +1016,20 @@ struct SeriesOps {
     int prec, K;
     double c, r;
     // a multi-step series (m > 0): accumulator q at accm + q * accm_stride takes
-    // the coefficients at coefm + q * coefm_stride, terms 0..Kq[q] (ascending);
-    // K is then Kq[m - 1] and acc / coef are not used
+    // the coefficients at coefm + q * coefm_stride, terms 0..kq[q] (device
+    // table; Kq on the host); the terms go to the PANEL slots of ring
+    // (ring_stride apart) and reach the accumulators term by term or in panel
+    // passes (series_core).  K is then kq[m - 1]; acc, coef, x0 and x1 are not
+    // used.
     int m = 0;
     cplx* accm = nullptr;
     long accm_stride = 0;
     const cplx* coefm = nullptr;
     int coefm_stride = 0;
-    const int* Kq = nullptr;
+    const int* kq = nullptr;
+    const int* Kq = nullptr;  // the same table on the host
+    cplx* ring = nullptr;
+    long ring_stride = 0;
 };
 
 
@@ -1039,10 +1047,17 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
     a.tri = TRI_FULL;
     a.x_stride = R; a.x_off = 0;
     a.out_stride = R; a.out_off = 0;
-    a.acc = o.acc;
-    a.coef = o.coef;
+    const bool panels = o.m > 0;
+    a.acc = panels ? nullptr : o.acc;
+    a.coef = panels ? nullptr : o.coef;
     a.M32 = o.H32;
-    if (o.m > 0) {
+    // a multi-step series: a term whose phase 2 is the symmetric block
+    // reduction adds T_k to the accumulators itself (small R: the update rides
+    // on loads the epilogue makes anyway); after any other term kind the
+    // accumulators take the terms in panel passes (one read of each T_k for
+    // all accumulators).  The first term decides; every term of a series runs
+    // the same kernel kind.
+    if (panels) {
         a.accm = o.accm;
         a.accm_stride = o.accm_stride;
         a.coefm = o.coefm;
@@ -1050,6 +1065,27 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
         a.m_lo = 0;
         a.m_hi = o.m;
     }
+    const PanelArgs pa{o.accm, o.accm_stride, o.coefm, o.coefm_stride, o.ring, o.ring_stride, o.phi, o.kq, o.m,
+                       (long)o.N * R};
+    auto slot = [&](int k) { return o.ring + (long)((k - 1) % PANEL) * o.ring_stride; };
+    int k_taken = -1;   // the last term the accumulators hold
+    int per_term = -1;  // -1: not known yet; 1: the terms accumulate; 0: panel passes
+    auto accumulated = [&](int k) -> int {
+        if (!panels) return DH_OK;
+        const int done = take_multi_done() ? 1 : 0;
+        if (per_term < 0) per_term = done;
+        if (done != per_term) {
+            dh::set_error("Chebyshev series: term %d changed kernel kind within a multi-step series", k);
+            return DH_ERR_STATE;
+        }
+        if (per_term) {
+            k_taken = k;
+        } else if (k % PANEL == 0 || k == o.K) {
+            launch_multi_panel(pa, k_taken + 1, k, st);
+            k_taken = k;
+        }
+        return DH_OK;
+    };
     const int prec = o.prec;
     // modes 1 and 3: the FP32 products read FP32 copies of their input (and mode
     // 3's residual products bf16 copies); the step's state is converted once,
@@ -1081,16 +1117,18 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
     // k = 1 (for K = 0 the device table holds a_1 = 0 and acc = a_0 phi)
     a.epi = EPI_CHEB_FIRST;
     a.x = o.phi;
-    a.out = o.x1;
+    a.out = panels ? slot(1) : o.x1;
     a.alpha = 1.0 / r;
     a.beta = -c / r;
     a.k = 1;
+    take_multi_done();
     if (int rc = term(a)) return rc;
+    if (int rc = accumulated(1)) return rc;
     const cplx* prev2 = o.phi;
-    cplx* prev = o.x1;
+    cplx* prev = a.out;
     for (int k = 2; k <= o.K; ++k) {
-        cplx* outb = (k == 2) ? o.x0 : const_cast<cplx*>(prev2);
-        if (o.m > 0)  // accumulators whose series has ended take no more terms
+        cplx* outb = panels ? slot(k) : (k == 2) ? o.x0 : const_cast<cplx*>(prev2);
+        if (panels)  // accumulators whose series has ended take no more terms
             while (a.m_lo < o.m - 1 && o.Kq[a.m_lo] < k) ++a.m_lo;
         a.epi = EPI_CHEB;
         a.x = prev;
@@ -1100,6 +1138,7 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
         a.beta = -2.0 * c / r;
         a.k = k;
         if (int rc = term(a)) return rc;
+        if (int rc = accumulated(k)) return rc;
         prev2 = prev;
         prev = outb;
     }
@@ -1459,6 +1498,10 @@ static int multi_coefficients(dh_system* s, double dt, int m, int* Kq) {
     constexpr int CS = KMAX_ALLOC + 2;
     if (!s->accm) {
         DALLOC(s->accm, (size_t)MULTI_MAX * acc_n);
+        // the ring's rows N..ld-1 are never written and must read as finite
+        // inputs of the next term
+        DALLOC(s->ring, (size_t)PANEL * acc_n);
+        DH_HIP(hipMemsetAsync(s->ring, 0, sizeof(cplx) * (size_t)PANEL * acc_n, st));
         DALLOC(s->coefm, (size_t)MULTI_MAX * CS);
         DALLOC(s->kq_d, (size_t)MULTI_MAX);
         DH_HIP(hipHostMalloc((void**)&s->coefm_h, sizeof(cplx) * MULTI_MAX * CS, hipHostMallocDefault));
@@ -1532,7 +1575,10 @@ static int multi_series(dh_system* s, double dt, int m, int* kmax) {
     o.accm_stride = acc_n;
     o.coefm = s->coefm;
     o.coefm_stride = CS;
+    o.kq = s->kq_d;
     o.Kq = Kq;
+    o.ring = s->ring;
+    o.ring_stride = acc_n;
     o.K = Kq[m - 1];
     if (int rc = series_core(o, st)) return rc;
     DH_HIP(hipGetLastError());
@@ -1609,19 +1655,10 @@ extern "C" int dh_propagate_steps(dh_system* s, int n_steps, double dt_ps, int r
                 DH_HIP(hipEventCreate(&e));
                 s->evpool.push_back(e);
             }
-        // steps through a term whose epilogue takes several accumulators run as
-        // multi-step series (up to MULTI_MAX steps per recurrence): FP64 with
-        // the split-K GEMM or a symmetric kernel, and the precision modes (the
-        // library GEMMs' reduction, the FP32-stored operator's symmetric kernel
-        // or split-K GEMM)
-        // The FP32-product modes (1, 3) read 6 bytes or fewer of operator per
-        // element against the accumulators' 32 N R bytes per term per active
-        // step (about 5 of 10 on average): below R = N / 50 the terms saved
-        // outweigh that traffic (N = 8192, R = 128: +20 %; N = 4096, R = 128:
-        // -3 to +1 %)
-        const bool multi_ok = s->steps_per_series != 1 &&
-                              (s->precision == 0 ? matvec_variant_multi_ok(matvec_variant_for(s->N, R, TRI_FULL))
-                                                 : !fp32_products(s) || 50L * R < s->N);
+        // the steps run as multi-step series (up to MULTI_MAX steps per
+        // recurrence) in every precision mode and with every term kernel: the
+        // accumulators take the terms in panel passes, outside the term kernels
+        const bool multi_ok = s->steps_per_series != 1;
         int n_timed = 0, chunk_terms = 0;
         long terms_timed = 0, launches = 0;
         for (int i = j; i < jend;) {
@@ -2250,8 +2287,17 @@ struct dh_batch {
     dh_context* ctx = nullptr;
     std::vector<dh_system*> sys;
     int W = 0, R = 0, n = 0;
-    enum { D_K1 = 0, D_K2, D_KODD, D_KEVEN, D_ENERGY, D_LBRA, D_LINVT, NDESC };
+    // descriptor kinds: terms 1, 2, odd and even k >= 3 of a one-step series,
+    // the outputs, and the terms k >= 3 of a multi-step series by k % PANEL
+    // (ring slots)
+    enum { D_K1 = 0, D_K2, D_KODD, D_KEVEN, D_ENERGY, D_LBRA, D_LINVT, D_RING0, NDESC = D_RING0 + PANEL };
     MatvecArgs* d_args = nullptr;  // [NDESC][n]
+    // panel passes of a multi-step series: task (member, 256-element block),
+    // members in term-count order (prefixP[j]: tasks of the first j)
+    PanelArgs* d_panel = nullptr;  // [n]
+    int *d_p_desc = nullptr, *d_p_blk = nullptr;
+    std::vector<int> prefixP;
+    std::vector<PanelArgs> h_panel;
     int *d_task_desc = nullptr, *d_task_blk = nullptr;
     NormTask* d_norms = nullptr;   // [2][n*R]: before, after
     CopyTask* d_copy = nullptr;    // [n]
@@ -2295,25 +2341,38 @@ static int g_batch_family[6] = {-1, -1, -1, -1, -1, -1};
 static std::mutex g_batch_family_mu;
 static int batch_family_slot(int R) { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : R == 32 ? 5 : 0; }
 
-static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k, hipStream_t st, bool multi = false) {
+static void batch_term_launch(dh_batch* b, bool sym, int kind, int active, int k, hipStream_t st) {
     const int n = b->n;
     if (sym)
         launch_sym_grouped(b->d_args + (size_t)kind * n, b->d_t1_desc, b->d_t1_idx, b->prefix1[active], b->d_t2_desc,
-                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, b->reduce_groups, st, multi);
+                           b->d_t2_idx, b->prefix2[active], b->d_sysS, b->R, k, b->reduce_groups, st);
     else
         launch_matvec_grouped(b->d_args + (size_t)kind * n, b->d_task_desc, b->d_task_blk, b->prefix[active], b->R, k,
                               st);
 }
 
 // Chebyshev terms of a batch's grouped members: those with K_s >= k are the
-// first `active` of `order`
+// first `active` of `order`.  A multi-step series (multi) writes its terms to
+// the members' rings and adds each PANEL of them to the accumulators of the
+// members that took any (every member's last panel ends at its own K_s).
 static void batch_series_launches(dh_batch* b, hipStream_t st, bool multi = false) {
     const int n = b->n;
-    int active = n;
+    int active = n, k_taken = -1;
     for (int k = 1; k <= b->Kmax; ++k) {
         while (active > 0 && b->Ks[b->order[active - 1]] < k) --active;
-        const int kind = k == 1 ? dh_batch::D_K1 : k == 2 ? dh_batch::D_K2 : (k & 1) ? dh_batch::D_KODD : dh_batch::D_KEVEN;
-        batch_term_launch(b, b->sym, kind, active, k, st, multi);
+        const int kind = k == 1   ? dh_batch::D_K1
+                         : k == 2 ? dh_batch::D_K2
+                         : multi  ? dh_batch::D_RING0 + k % PANEL
+                         : (k & 1) ? dh_batch::D_KODD
+                                   : dh_batch::D_KEVEN;
+        batch_term_launch(b, b->sym, kind, active, k, st);
+        if (multi && (k % PANEL == 0 || k == b->Kmax)) {
+            const int k0 = k_taken + 1;
+            int took = n;
+            while (took > 0 && b->Ks[b->order[took - 1]] < k0) --took;
+            launch_multi_panel_grouped(b->d_panel, b->d_p_desc, b->d_p_blk, b->prefixP[took], k0, k, st);
+            k_taken = k;
+        }
     }
 }
 
@@ -2380,8 +2439,9 @@ extern "C" int dh_batch_destroy(dh_batch* b) {
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
-    void* ptrs[] = {b->d_args,  b->d_task_desc, b->d_task_blk, b->d_norms, b->d_copy, b->d_obs,
-                    b->d_nval,  b->d_t1_desc,   b->d_t1_idx,   b->d_t2_desc, b->d_t2_idx, b->d_sysS};
+    void* ptrs[] = {b->d_args,  b->d_task_desc, b->d_task_blk, b->d_norms,  b->d_copy,  b->d_obs,  b->d_nval,
+                    b->d_t1_desc, b->d_t1_idx,  b->d_t2_desc,  b->d_t2_idx, b->d_sysS, b->d_panel, b->d_p_desc,
+                    b->d_p_blk};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete b;
@@ -2425,6 +2485,11 @@ extern "C" int dh_batch_create(dh_context* ctx, int n, dh_system* const* systems
     BALLOC(b->d_copy, (size_t)n);
     BALLOC(b->d_obs, (size_t)n * W);
     BALLOC(b->d_nval, (size_t)2 * n * R);
+    long ptasks = 0;
+    for (auto* s : b->sys) ptasks += ((long)s->N * R + 255) / 256;
+    BALLOC(b->d_panel, (size_t)n);
+    BALLOC(b->d_p_desc, (size_t)ptasks);
+    BALLOC(b->d_p_blk, (size_t)ptasks);
     b->sym_ok = sym_supported(R, TRI_FULL);
     if (b->sym_ok) {
         long t1 = 0, t2 = 0;
@@ -2494,10 +2559,44 @@ static MatvecArgs batch_desc(dh_system* s, int kind) {
             a.M = s->Q1; a.R = W; a.tri = TRI_UPPER; a.epi = EPI_PLAIN;
             a.x = s->phi; a.x_stride = R; a.out = s->out; a.out_stride = 3 * W;
             break;
-        default:  // D_LINVT
+        case dh_batch::D_LINVT:
             a.M = s->Q2; a.R = R; a.tri = TRI_LOWER; a.epi = EPI_PLAIN;
             a.x = s->phi; a.x_stride = R; a.out = s->out; a.out_stride = 3 * W; a.out_off = W;
             break;
+        default:  // D_RING0 + k % PANEL, k >= 3 (set up by dh_batch_propagate_steps)
+            a.M = s->Hth; a.R = R; a.tri = TRI_FULL; a.epi = EPI_CHEB;
+            a.alpha = 2.0 / r; a.beta = -2.0 * c / r;
+            a.x_stride = R; a.out_stride = R;
+            break;
+    }
+    return a;
+}
+
+// a multi-step series' term descriptors (kinds D_K1, D_K2 and the ring kinds):
+// T_k in ring slot (k - 1) % PANEL, no single accumulator
+static MatvecArgs batch_ring_desc(dh_system* s, int kind) {
+    const int R = 2 * s->W;
+    MatvecArgs a = batch_desc(s, kind);
+    const double c = 0.5 * (s->emax + s->emin), r = 0.5 * (s->emax - s->emin) * s->widen;
+    const double f = kind == dh_batch::D_K1 ? 1.0 : 2.0;
+    a.alpha = f / r;
+    a.beta = -f * c / r;
+    a.acc = nullptr;
+    a.coef = nullptr;
+    auto slot = [&](int k) { return s->ring + (long)((k - 1) % PANEL) * s->ld * R; };
+    if (kind == dh_batch::D_K1) {
+        a.x = s->phi;
+        a.out = slot(1);
+    } else if (kind == dh_batch::D_K2) {
+        a.x = slot(1);
+        a.x2 = s->phi;
+        a.out = slot(2);
+    } else {
+        const int j = kind - dh_batch::D_RING0;
+        const int k = j >= 3 ? j : j + PANEL;  // a term k >= 3 with k % PANEL == j
+        a.x = slot(k - 1);
+        a.x2 = slot(k - 2);
+        a.out = slot(k);
     }
     return a;
 }
@@ -2620,6 +2719,23 @@ static int batch_retable(dh_batch* b, const std::vector<int>& Ks, const std::vec
             DH_HIP(hipMemcpyAsync(b->d_t2_desc, d2.data(), sizeof(int) * d2.size(), hipMemcpyHostToDevice, st));
             DH_HIP(hipMemcpyAsync(b->d_t2_idx, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice, st));
         }
+    }
+    {  // panel passes: every grouped member, 256 elements of rows < N per task
+        std::vector<int> pd, pb;
+        b->prefixP.assign(1, 0);
+        for (int q : b->order) {
+            if (!solo[q])
+                for (long blk = 0; blk < ((long)b->sys[q]->N * R + 255) / 256; ++blk) {
+                    pd.push_back(q);
+                    pb.push_back((int)blk);
+                }
+            b->prefixP.push_back((int)pd.size());
+        }
+        if (!pd.empty()) {
+            DH_HIP(hipMemcpyAsync(b->d_p_desc, pd.data(), sizeof(int) * pd.size(), hipMemcpyHostToDevice, st));
+            DH_HIP(hipMemcpyAsync(b->d_p_blk, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
+        }
+        DH_HIP(hipStreamSynchronize(st));  // pd, pb are local
     }
     b->Kmax = 0;
     for (int i = 0; i < n; ++i)
@@ -2744,12 +2860,12 @@ extern "C" int dh_batch_propagate(dh_batch* b, double t_init, double t_max, int*
 }
 
 // n_steps steps of dt_ps for every member, up to MULTI_MAX steps per
-// recurrence (the multi-step series of dh_propagate_steps, grouped): each
-// member's accumulators take their own coefficient rows, and the grouped
-// phase-2 kernel finds a member's first active accumulator from its term table.
+// recurrence (the multi-step series of dh_propagate_steps, grouped): the
+// grouped terms write each member's ring, and grouped panel passes add them to
+// the member's accumulators with its own coefficient rows and term table.
 // One norm check, commit and output pass per recurrence; the outputs are those
-// of the final state.  Batches whose members are not all FP64, or whose term
-// family is the dense one, step one step at a time (dh_batch_propagate).
+// of the final state.  Batches whose members are not all FP64 step one step at
+// a time (dh_batch_propagate).
 extern "C" int dh_batch_propagate_steps(dh_batch* b, int n_steps, double dt_ps, int* max_terms) {
     CHECK_ARG(b && n_steps > 0 && dt_ps > 0.0, "dh_batch_propagate_steps: need a batch, n_steps > 0 and dt > 0");
     DH_HIP(hipSetDevice(b->ctx->device));
@@ -2766,7 +2882,7 @@ extern "C" int dh_batch_propagate_steps(dh_batch* b, int n_steps, double dt_ps, 
         kmax_seen = k;
         done = 1;
     }
-    if (!all_fp64 || !b->sym) {
+    if (!all_fp64) {
         for (; done < n_steps; ++done) {
             int k = 0;
             if (int rc = dh_batch_propagate(b, 0.0, dt_ps, &k)) return rc;
@@ -2801,30 +2917,32 @@ extern "C" int dh_batch_propagate_steps(dh_batch* b, int n_steps, double dt_ps, 
         std::vector<char> solo(n, 0);
         if (Ks != b->Ks || solo != b->solo || b->prefix.empty())
             if (int rc = batch_retable(b, Ks, solo, st)) return rc;
-        // term descriptors with the members' accumulators; the output
-        // descriptors read phi, which the commit below fills
+        // term descriptors over the members' rings; the output descriptors
+        // read phi, which the commit below fills
         std::vector<MatvecArgs> args((size_t)dh_batch::NDESC * n);
+        std::vector<PanelArgs> pargs(n);
         for (int kind = 0; kind < dh_batch::NDESC; ++kind)
             for (int i = 0; i < n; ++i) {
                 dh_system* s = b->sys[i];
-                MatvecArgs a = batch_desc(s, kind);
-                if (kind <= dh_batch::D_KEVEN) {
-                    const double c = 0.5 * (s->emax + s->emin), r = 0.5 * (s->emax - s->emin) * s->widen;
-                    const double f = kind == dh_batch::D_K1 ? 1.0 : 2.0;
-                    a.alpha = f / r;
-                    a.beta = -f * c / r;
-                    a.acc = nullptr;
-                    a.coef = nullptr;
-                    a.accm = s->accm;
-                    a.accm_stride = (long)s->ld * R;
-                    a.coefm = s->coefm;
-                    a.coefm_stride = (int)cs;
-                    a.m_lo = 0;
-                    a.m_hi = m;
-                    a.kq = s->kq_d;
-                }
-                args[(size_t)kind * n + i] = a;
+                const bool term = kind == dh_batch::D_K1 || kind == dh_batch::D_K2 || kind >= dh_batch::D_RING0;
+                args[(size_t)kind * n + i] = term ? batch_ring_desc(s, kind) : batch_desc(s, kind);
             }
+        for (int i = 0; i < n; ++i) {
+            dh_system* s = b->sys[i];
+            PanelArgs p{};
+            p.accm = s->accm;
+            p.accm_stride = (long)s->ld * R;
+            p.coefm = s->coefm;
+            p.coefm_stride = (int)cs;
+            p.ring = s->ring;
+            p.ring_stride = (long)s->ld * R;
+            p.phi = s->phi;
+            p.kq = s->kq_d;
+            p.m = m;
+            p.n = (long)s->N * R;
+            pargs[i] = p;
+        }
+        if (int rc = upload_if_changed(b->h_panel, pargs, b->d_panel, st)) return rc;
         std::vector<NormTask> nt((size_t)2 * n * R);
         for (int i = 0; i < n; ++i) {
             dh_system* s = b->sys[i];

@@ -27,6 +27,24 @@
 
 namespace dh {
 
+// The accumulator of a one-step series for output element oi, given T_k and
+// the term's input x: EPI_CHEB: acc += a_k T_k; EPI_CHEB_FIRST:
+// acc = a_0 x + a_1 T_1.  A null a.acc skips it (the recurrences of a
+// multi-step series, whose accumulators take the terms in panels).
+__device__ __forceinline__ void acc_update(const MatvecArgs& a, long oi, cplx xv, cplx T) {
+    if (!a.acc) return;
+    if (a.epi == EPI_CHEB) {
+        const cplx add = cmul(a.coef[a.k], T);
+        cplx acc = a.acc[oi];
+        acc.re += add.re;
+        acc.im += add.im;
+        a.acc[oi] = acc;
+    } else {
+        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
+        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+    }
+}
+
 constexpr int MV_WAVES = 4;
 
 template <int R, int CPW>
@@ -156,15 +174,8 @@ __global__ __launch_bounds__(256) void matvec_kernel(MatvecArgs a) {
                 const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
                 T.re -= x2.re;
                 T.im -= x2.im;
-                const cplx add = cmul(a.coef[a.k], T);
-                cplx acc = a.acc[oi];
-                acc.re += add.re;
-                acc.im += add.im;
-                a.acc[oi] = acc;
-            } else {
-                const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-                a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
             }
+            acc_update(a, oi, xv, T);
             a.out[oi] = T;
         }
     }
@@ -328,15 +339,8 @@ if constexpr (PIPE == 0) {
         const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
         T.re -= x2.re;
         T.im -= x2.im;
-        const cplx add = cmul(a.coef[a.k], T);
-        cplx acc = a.acc[oi];
-        acc.re += add.re;
-        acc.im += add.im;
-        a.acc[oi] = acc;
-    } else {
-        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
     }
+    acc_update(a, oi, xv, T);
     a.out[oi] = T;
 }
 
@@ -451,15 +455,8 @@ __global__ void matvec_reduce_epilogue_kernel(MatvecArgs a) {
         const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
         T.re -= x2.re;
         T.im -= x2.im;
-        const cplx add = cmul(a.coef[a.k], T);
-        cplx acc = a.acc[oi];
-        acc.re += add.re;
-        acc.im += add.im;
-        a.acc[oi] = acc;
-    } else {
-        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
     }
+    acc_update(a, oi, xv, T);
     a.out[oi] = T;
 }
 
@@ -565,15 +562,8 @@ __global__ __launch_bounds__(256) void cheb_gemm_kernel(MatvecArgs a) {
             const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
             T.re -= x2.re;
             T.im -= x2.im;
-            const cplx add = cmul(a.coef[a.k], T);
-            cplx ac = a.acc[oi];
-            ac.re += add.re;
-            ac.im += add.im;
-            a.acc[oi] = ac;
-        } else {
-            const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-            a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
         }
+        acc_update(a, oi, xv, T);
         a.out[oi] = T;
     }
 }
@@ -689,15 +679,8 @@ __global__ __launch_bounds__(256) void cheb_gemm2_kernel(MatvecArgs a) {
             const cplx x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
             Tk.re -= x2.re;
             Tk.im -= x2.im;
-            const cplx add = cmul(a.coef[a.k], Tk);
-            cplx ac = a.acc[oi];
-            ac.re += add.re;
-            ac.im += add.im;
-            a.acc[oi] = ac;
-        } else {
-            const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], Tk);
-            a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
         }
+        acc_update(a, oi, xv, Tk);
         a.out[oi] = Tk;
     }
 }
@@ -710,61 +693,11 @@ __host__ __device__ inline unsigned short bf16_rne(float f) {
     return (unsigned short)(u >> 16);
 }
 
-// The accumulators of a multi-step series (a.accm): term k adds
-// coefm[q][k] T_k to accumulator q (each with its own time's coefficients, the
-// same T_k); the first term sets them to coefm[q][0] x + coefm[q][1] T_1.
-// am holds accumulators m_lo .. m_hi - 1 as loaded before (EPI_CHEB); all are
-// read before any is written.
-__device__ __forceinline__ void multi_load(const MatvecArgs& a, long oi, cplx* am) {
-#pragma unroll
-    for (int u = 0; u < MULTI_MAX; ++u)
-        if (a.m_lo + u < a.m_hi) am[u] = a.accm[(long)(a.m_lo + u) * a.accm_stride + oi];
-}
-
-// the coefficients a_k of accumulators m_lo .. m_hi - 1 for this term (EPI_CHEB)
-__device__ __forceinline__ void multi_coefs(const MatvecArgs& a, cplx* cm) {
-#pragma unroll
-    for (int u = 0; u < MULTI_MAX; ++u)
-        if (a.m_lo + u < a.m_hi) cm[u] = a.coefm[(long)(a.m_lo + u) * a.coefm_stride + a.k];
-}
-
-// cm: the coefficients as multi_coefs loads them (EPI_CHEB; registers or LDS),
-// read before the first store; EPI_CHEB_FIRST (one term per series) reads its
-// two coefficients per accumulator directly
-__device__ __forceinline__ void multi_store(const MatvecArgs& a, long oi, cplx xv, cplx T, const cplx* am,
-                                            const cplx* cm) {
-    if (a.epi == EPI_CHEB) {
-#pragma unroll
-        for (int u = 0; u < MULTI_MAX; ++u) {
-            const int q = a.m_lo + u;
-            if (q < a.m_hi) {
-                const cplx add = cmul(cm[u], T);
-                a.accm[(long)q * a.accm_stride + oi] = cplx{am[u].re + add.re, am[u].im + add.im};
-            }
-        }
-    } else {
-        for (int q = a.m_lo; q < a.m_hi; ++q) {
-            const cplx* cq = a.coefm + (long)q * a.coefm_stride;
-            const cplx t0 = cmul(cq[0], xv), t1 = cmul(cq[1], T);
-            a.accm[(long)q * a.accm_stride + oi] = cplx{t0.re + t1.re, t0.im + t1.im};
-        }
-    }
-}
-
-__device__ __forceinline__ void multi_accumulate(const MatvecArgs& a, long oi, cplx xv, cplx T) {
-    cplx am[MULTI_MAX], cm[MULTI_MAX];
-    if (a.epi == EPI_CHEB) {
-        multi_load(a, oi, am);
-        multi_coefs(a, cm);
-    }
-    multi_store(a, oi, xv, T, am, cm);
-}
-
 // Chebyshev epilogue for output row i, column r with y = (H x)_ir:
 //   EPI_PLAIN:       out = y
 //   EPI_CHEB_FIRST:  T = alpha y + beta x; acc = a_0 x + a_1 T; out = T
 //   EPI_CHEB:        T = alpha y + beta x - x2; acc += a_k T; out = T
-// (with a.accm, the accumulators of the multi-step series instead of acc)
+// (a null acc: the recurrence of a multi-step series, out = T only)
 __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r, cplx y) {
     const long oi = (long)i * a.out_stride + a.out_off + r;
     if (a.epi == EPI_PLAIN) {
@@ -778,18 +711,7 @@ __device__ __forceinline__ void term_epilogue(const MatvecArgs& a, int i, int r,
         T.re -= x2.re;
         T.im -= x2.im;
     }
-    if (a.accm) {
-        multi_accumulate(a, oi, xv, T);
-    } else if (a.epi == EPI_CHEB) {
-        const cplx add = cmul(a.coef[a.k], T);
-        cplx acc = a.acc[oi];
-        acc.re += add.re;
-        acc.im += add.im;
-        a.acc[oi] = acc;
-    } else {
-        const cplx t0 = cmul(a.coef[0], xv), t1 = cmul(a.coef[1], T);
-        a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
-    }
+    acc_update(a, oi, xv, T);
     a.out[oi] = T;
     if (a.out32) {  // the next term's FP32 operand: the value its GEMM would round on load
         a.out32[2 * oi] = (float)T.re;
@@ -813,7 +735,7 @@ __device__ __forceinline__ EpiOperands epilogue_operands(const MatvecArgs& a, in
     o.xv = a.x[(long)i * a.x_stride + a.x_off + r];
     if (a.epi == EPI_CHEB) {
         o.x2 = a.x2[(long)i * a.x_stride + a.x_off + r];
-        if (!a.accm) {
+        if (a.acc) {
             o.acc = a.acc[oi];
             o.ck = a.coef[a.k];
         }
@@ -821,74 +743,20 @@ __device__ __forceinline__ EpiOperands epilogue_operands(const MatvecArgs& a, in
     return o;
 }
 
-// T_k of the epilogue (EPI_CHEB or EPI_CHEB_FIRST) from y and its operands
-__device__ __forceinline__ cplx term_value_pre(const MatvecArgs& a, cplx y, const EpiOperands& o) {
-    cplx T{a.alpha * y.re + a.beta * o.xv.re, a.alpha * y.im + a.beta * o.xv.im};
-    if (a.epi == EPI_CHEB) {
-        T.re -= o.x2.re;
-        T.im -= o.x2.im;
-    }
-    return T;
-}
-
-// Accumulator updates of a multi-step series spread over the workgroup:
-// group g (64 threads, one element each) owns accumulators m_lo + g + v G,
-// v < QP, prefetched with their coefficients at the start (multi_spread_load)
-// and updated with T_k from LDS at the end (multi_spread_store), so that no
-// thread updates more than QP of them.
-template <int G>
-struct MultiSpread {
-    static constexpr int QP = (MULTI_MAX + G - 1) / G;
-    cplx am[QP], cq[QP];
-};
-
-// the first accumulator taking term a.k: a.m_lo, or for a batch member (a.kq)
-// the first whose series reaches k (the last one always does)
-__device__ __forceinline__ int multi_first(const MatvecArgs& a) {
-    if (!a.kq) return a.m_lo;
-    int q = 0;
-    while (q < a.m_hi - 1 && a.kq[q] < a.k) ++q;
-    return q;
-}
-
-template <int G>
-__device__ __forceinline__ void multi_spread_load(const MatvecArgs& a, int m_lo, int g, long oi, bool valid,
-                                                  MultiSpread<G>& ms) {
-#pragma unroll
-    for (int v = 0; v < MultiSpread<G>::QP; ++v) {
-        const int q = m_lo + g + v * G;
-        if (valid && q < a.m_hi) {
-            ms.am[v] = a.accm[(long)q * a.accm_stride + oi];
-            ms.cq[v] = a.coefm[(long)q * a.coefm_stride + a.k];
-        }
-    }
-}
-
-template <int G>
-__device__ __forceinline__ void multi_spread_store(const MatvecArgs& a, int m_lo, int g, long oi, bool valid,
-                                                   const MultiSpread<G>& ms, cplx T) {
-#pragma unroll
-    for (int v = 0; v < MultiSpread<G>::QP; ++v) {
-        const int q = m_lo + g + v * G;
-        if (valid && q < a.m_hi) {
-            const cplx add = cmul(ms.cq[v], T);
-            a.accm[(long)q * a.accm_stride + oi] = cplx{ms.am[v].re + add.re, ms.am[v].im + add.im};
-        }
-    }
-}
-
-__device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, int r, cplx y, const EpiOperands& o) {
+// returns what it writes to out (T_k, or y for EPI_PLAIN)
+__device__ __forceinline__ cplx term_epilogue_pre(const MatvecArgs& a, int i, int r, cplx y, const EpiOperands& o) {
     const long oi = (long)i * a.out_stride + a.out_off + r;
     if (a.epi == EPI_PLAIN) {
         a.out[oi] = y;
-        return;
+        return y;
     }
     cplx T{a.alpha * y.re + a.beta * o.xv.re, a.alpha * y.im + a.beta * o.xv.im};
     if (a.epi == EPI_CHEB) {
         T.re -= o.x2.re;
         T.im -= o.x2.im;
     }
-    if (a.epi == EPI_CHEB) {
+    if (!a.acc) {
+    } else if (a.epi == EPI_CHEB) {
         const cplx add = cmul(o.ck, T);
         a.acc[oi] = cplx{o.acc.re + add.re, o.acc.im + add.im};
     } else {
@@ -896,6 +764,103 @@ __device__ __forceinline__ void term_epilogue_pre(const MatvecArgs& a, int i, in
         a.acc[oi] = cplx{t0.re + t1.re, t0.im + t1.im};
     }
     a.out[oi] = T;
+    return T;
+}
+
+// ------------------------------------------------------------------ multi-step series: panels
+// Accumulator q of a multi-step series takes sum_k coefm[q][k] T_k.  The
+// recurrence writes T_k (k >= 1) to ring slot (k - 1) % PANEL and nothing
+// else; after every PANEL terms (and after the last) one pass adds the
+// panel's terms k0..k1 to the accumulators, in term order, reading each T_k
+// once for all accumulators (T_0 = phi, in the first panel, which starts the
+// accumulators from zero).  Accumulator q takes terms up to kq[q]; the last
+// accumulator's kq is the member's series length.
+// The first accumulator taking any of terms k0.. (q_lo) and the panel's last
+// term for this member (kend <= k1): kq is non-decreasing, so q_lo counts the
+// accumulators that end before k0.  The table is read in one round of
+// independent loads.
+__device__ __forceinline__ void panel_range(const PanelArgs& p, int k0, int k1, int& q_lo, int& kend) {
+    int kv[MULTI_MAX];
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u) kv[u] = u < p.m ? p.kq[u] : 0;
+    q_lo = 0;
+    kend = k1;
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u) {
+        if (u < p.m - 1 && kv[u] < k0) ++q_lo;
+        if (u == p.m - 1) kend = min(k1, kv[u]);
+    }
+}
+
+// the panel's coefficients: Cs[u * (PANEL + 1) + j] = coefm[q_lo + u][k0 + j]
+// (zero past the member's accumulators or its last term), one load round for
+// the workgroup; the caller synchronises before reading them
+constexpr int PANEL_CS = MULTI_MAX * (PANEL + 1);
+__device__ __forceinline__ void panel_coefs(const PanelArgs& p, int q_lo, int k0, int kend, cplx* Cs) {
+    for (int x = threadIdx.x; x < PANEL_CS; x += blockDim.x) {
+        const int u = x / (PANEL + 1), j = x % (PANEL + 1);
+        const int q = q_lo + u;
+        Cs[x] = (q < p.m && k0 + j <= kend) ? p.coefm[(long)q * p.coefm_stride + k0 + j] : cplx{0.0, 0.0};
+    }
+}
+
+__device__ __forceinline__ void panel_body(const PanelArgs& p, int k0, int k1, long e) {
+    __shared__ cplx Cs[PANEL_CS];
+    int q, kend;
+    panel_range(p, k0, k1, q, kend);
+    panel_coefs(p, q, k0, kend, Cs);
+    __syncthreads();
+    if (e >= p.n || kend < k0) return;
+    cplx T[PANEL + 1];
+#pragma unroll
+    for (int j = 0; j <= PANEL; ++j)
+        if (k0 + j <= kend) {
+            const int k = k0 + j;
+            T[j] = k == 0 ? p.phi[e] : p.ring[(long)((k - 1) % PANEL) * p.ring_stride + e];
+        }
+    // every accumulator is loaded before any is stored, so that the loads
+    // overlap (the stores could alias them as far as the compiler knows)
+    cplx A[MULTI_MAX];
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u)
+        if (q + u < p.m) A[u] = k0 == 0 ? cplx{0.0, 0.0} : p.accm[(long)(q + u) * p.accm_stride + e];
+#pragma unroll
+    for (int u = 0; u < MULTI_MAX; ++u)
+        if (q + u < p.m) {
+            cplx acc = A[u];
+#pragma unroll
+            for (int j = 0; j <= PANEL; ++j)
+                if (k0 + j <= kend) {
+                    const cplx add = cmul(Cs[u * (PANEL + 1) + j], T[j]);
+                    acc.re += add.re;
+                    acc.im += add.im;
+                }
+            p.accm[(long)(q + u) * p.accm_stride + e] = acc;
+        }
+}
+
+__global__ __launch_bounds__(256) void multi_panel_kernel(PanelArgs p, int k0, int k1) {
+    panel_body(p, k0, k1, blockIdx.x * 256L + threadIdx.x);
+}
+
+// batch members: task i is (member task_desc[i], 256-element block task_blk[i])
+__global__ __launch_bounds__(256) void multi_panel_grouped_kernel(const PanelArgs* __restrict__ args,
+                                                                  const int* __restrict__ task_desc,
+                                                                  const int* __restrict__ task_blk, int k0, int k1) {
+    const PanelArgs p = args[task_desc[blockIdx.x]];
+    panel_body(p, k0, k1, task_blk[blockIdx.x] * 256L + threadIdx.x);
+}
+
+void launch_multi_panel(const PanelArgs& p, int k0, int k1, hipStream_t st) {
+    const unsigned blocks = (unsigned)((p.n + 255) / 256);
+    if (blocks) hipLaunchKernelGGL(multi_panel_kernel, dim3(blocks), dim3(256), 0, st, p, k0, k1);
+}
+
+void launch_multi_panel_grouped(const PanelArgs* d_args, const int* task_desc, const int* task_blk, int ntask,
+                                int k0, int k1, hipStream_t st) {
+    if (ntask > 0)
+        hipLaunchKernelGGL(multi_panel_grouped_kernel, dim3(ntask), dim3(256), 0, st, d_args, task_desc, task_blk,
+                           k0, k1);
 }
 
 // Split-K multi-RHS term GEMM: Y = H X with H N x N (FP64, or its FP32 copy
@@ -1389,10 +1354,16 @@ constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per sourc
 static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
 __host__ __device__ inline int sym_reduce_groups(int nb) { return nb > 32 ? 16 : 4; }
 
+// MULTI: a multi-step series' per-term accumulation (a.accm): T_k goes to
+// accumulators m_lo..m_hi-1 with their own coefficients.  Group g of the
+// workgroup owns accumulators m_lo + g + v G (v < QP), prefetched with their
+// coefficients alongside the epilogue's operands, and updates them with T_k
+// from LDS at the end, so that no thread updates more than QP of them.
 template <int R, int G, bool MULTI = false>
 __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int chunk) {
     __shared__ cplx red[G * 64];
     __shared__ cplx Ts[MULTI ? 64 : 1];
+    constexpr int QP = (MULTI_MAX + G - 1) / G;
     const int nb = a.ld / TILE;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
     const int seg0 = sym_segment_start(B, S);
@@ -1403,14 +1374,22 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     const long pcol = (long)nb * (nb - 1) / 2 * E;
     const int i = B * TILE + e / R;
     const long oi = (long)i * a.out_stride + a.out_off + e % R;
+    const bool valid = i < a.N;
     // the epilogue's operands do not depend on the partials: fetch them first
-    // (a multi-step series' accumulators: every group its own share)
     EpiOperands ep{};
-    if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
-    MultiSpread<G> ms;
+    if (t < 64 && valid) ep = epilogue_operands(a, i, e % R);
     const bool spread = MULTI && a.epi == EPI_CHEB;
-    const int m_lo = MULTI ? multi_first(a) : 0;
-    if (spread) multi_spread_load<G>(a, m_lo, g, oi, i < a.N, ms);
+    cplx am[MULTI ? QP : 1], cm[MULTI ? QP : 1];
+    if (spread) {
+#pragma unroll
+        for (int v = 0; v < QP; ++v) {
+            const int q = a.m_lo + g + v * G;
+            if (valid && q < a.m_hi) {
+                am[v] = a.accm[(long)q * a.accm_stride + oi];
+                cm[v] = a.coefm[(long)q * a.coefm_stride + a.k];
+            }
+        }
+    }
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -1437,42 +1416,62 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
             y.re += red[gg * 64 + t].re;
             y.im += red[gg * 64 + t].im;
         }
-        if (!MULTI) {
-            if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
-        } else if (i < a.N) {
-            const cplx T = term_value_pre(a, y, ep);
-            a.out[oi] = T;
-            Ts[t] = T;
-            if (a.epi == EPI_CHEB_FIRST) multi_store(a, oi, ep.xv, T, nullptr, nullptr);
+        if (valid) {
+            const cplx T = term_epilogue_pre(a, i, e % R, y, ep);
+            if (MULTI) {
+                Ts[t] = T;
+                if (a.epi == EPI_CHEB_FIRST)  // one term starts every accumulator
+                    for (int q = a.m_lo; q < a.m_hi; ++q) {
+                        const cplx* cq = a.coefm + (long)q * a.coefm_stride;
+                        const cplx t0 = cmul(cq[0], ep.xv), t1 = cmul(cq[1], T);
+                        a.accm[(long)q * a.accm_stride + oi] = cplx{t0.re + t1.re, t0.im + t1.im};
+                    }
+            }
         }
     }
     if (spread) {
         __syncthreads();  // T_k of the 64 elements in LDS
-        multi_spread_store<G>(a, m_lo, g, oi, i < a.N, ms, Ts[t & 63]);
+        const cplx T = Ts[t & 63];
+#pragma unroll
+        for (int v = 0; v < QP; ++v) {
+            const int q = a.m_lo + g + v * G;
+            if (valid && q < a.m_hi) {
+                const cplx add = cmul(cm[v], T);
+                a.accm[(long)q * a.accm_stride + oi] = cplx{am[v].re + add.re, am[v].im + add.im};
+            }
+        }
     }
 }
 
-// MULTI: the epilogue of a multi-step series (a.accm), compiled apart so that
-// the other launches keep their registers
+// MULTI: compiled apart so that the other launches keep their registers
 template <int R, int G, bool MULTI>
 __global__ __launch_bounds__(G * 64) void sym_reduce_kernel(MatvecArgs a, int S) {
     sym_reduce_body<R, G, MULTI>(a, S, blockIdx.x, blockIdx.y);
 }
 
+static thread_local bool t_multi_done = false;
+bool take_multi_done() {
+    const bool f = t_multi_done;
+    t_multi_done = false;
+    return f;
+}
+
 template <int R>
 static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st) {
+    const bool multi = a.accm != nullptr;
     if (sym_reduce_groups(nb) == 16) {
-        if (a.accm) hipLaunchKernelGGL((sym_reduce_kernel<R, 16, true>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
+        if (multi) hipLaunchKernelGGL((sym_reduce_kernel<R, 16, true>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
         else hipLaunchKernelGGL((sym_reduce_kernel<R, 16, false>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
     } else {
-        if (a.accm) hipLaunchKernelGGL((sym_reduce_kernel<R, 4, true>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
+        if (multi) hipLaunchKernelGGL((sym_reduce_kernel<R, 4, true>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
         else hipLaunchKernelGGL((sym_reduce_kernel<R, 4, false>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
     }
+    if (multi) t_multi_done = true;
 }
 
 // Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),
 // with the system's segment length sys_S[task_desc[i]]
-template <int R, int G, bool MULTI = false>
+template <int R, int G>
 __global__ __launch_bounds__(G * 64) void sym_reduce_grouped_kernel(const MatvecArgs* __restrict__ args,
                                                                     const int* __restrict__ task_desc,
                                                                     const int* __restrict__ task_idx,
@@ -1480,7 +1479,7 @@ __global__ __launch_bounds__(G * 64) void sym_reduce_grouped_kernel(const Matvec
     const int d = task_desc[blockIdx.x], v = task_idx[blockIdx.x];
     MatvecArgs a = args[d];
     a.k = k;
-    sym_reduce_body<R, G, MULTI>(a, sys_S[d], v / R, v % R);
+    sym_reduce_body<R, G>(a, sys_S[d], v / R, v % R);
 }
 
 template <int R>
@@ -1683,8 +1682,7 @@ int sym_term_segments(int ld, int* S) {
 }
 
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st,
-                        bool multi) {
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st) {
     if (n1 > 0) switch (R) {
             case 1: hipLaunchKernelGGL(symm_tiles_grouped_kernel<1>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
             case 2: hipLaunchKernelGGL(symm_tiles_grouped_kernel<2>, dim3(n1), dim3(256), 0, st, d_args, t1_desc, t1_idx, sys_S); break;
@@ -1693,15 +1691,9 @@ void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int*
             default: break;
         }
     if (n2 <= 0) return;
-#define DH_SYM_RED(RR, GG)                                                                                             \
-    do {                                                                                                               \
-        if (multi)                                                                                                     \
-            hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG, true>), dim3(n2), dim3(GG * 64), 0, st, d_args,      \
-                               t2_desc, t2_idx, sys_S, k);                                                             \
-        else                                                                                                           \
-            hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG, false>), dim3(n2), dim3(GG * 64), 0, st, d_args,     \
-                               t2_desc, t2_idx, sys_S, k);                                                             \
-    } while (0)
+#define DH_SYM_RED(RR, GG)                                                                                 \
+    hipLaunchKernelGGL((sym_reduce_grouped_kernel<RR, GG>), dim3(n2), dim3(GG * 64), 0, st, d_args, t2_desc, t2_idx, \
+                       sys_S, k)
     if (reduce_groups == 16) switch (R) {
             case 1: DH_SYM_RED(1, 16); break;
             case 2: DH_SYM_RED(2, 16); break;
@@ -2004,10 +1996,9 @@ __global__ __launch_bounds__(256) void symm2_tiles_kernel(const double* __restri
 // Phase 2 of variant 19: block B adds the row partials of super-tiles
 // (B / 2, 0..B / 2) (row B % 2 of each) and the column partials Pcol[P^2 + B]
 // for the super-tile rows P below B, in a fixed order, then the epilogue.
-template <int R, bool MULTI>
+template <int R>
 __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecArgs a) {
     __shared__ cplx red[SYM_G * 64];
-    __shared__ cplx Ts[MULTI ? 64 : 1];
     const int nb = a.ld / TILE, npr = (nb + 1) / 2;
     const int B = blockIdx.x, chunk = blockIdx.y;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
@@ -2022,10 +2013,6 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
     const long oi = (long)i * a.out_stride + a.out_off + e % R;
     EpiOperands ep{};
     if (t < 64 && i < a.N) ep = epilogue_operands(a, i, e % R);
-    MultiSpread<SYM_G> ms;
-    const bool spread = MULTI && a.epi == EPI_CHEB;
-    const int m_lo = MULTI ? multi_first(a) : 0;
-    if (spread) multi_spread_load<SYM_G>(a, m_lo, g, oi, i < a.N, ms);
     double sr = 0.0, si = 0.0;
     for (int base = s0; base < s1; base += SYM_MAXS) {
         cplx p[SYM_MAXS];
@@ -2052,18 +2039,7 @@ __global__ __launch_bounds__(SYM_REDUCE_THREADS) void sym2_reduce_kernel(MatvecA
             y.re += red[gg * 64 + t].re;
             y.im += red[gg * 64 + t].im;
         }
-        if (!MULTI) {
-            if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
-        } else if (i < a.N) {
-            const cplx T = term_value_pre(a, y, ep);
-            a.out[oi] = T;
-            Ts[t] = T;
-            if (a.epi == EPI_CHEB_FIRST) multi_store(a, oi, ep.xv, T, nullptr, nullptr);
-        }
-    }
-    if (spread) {
-        __syncthreads();  // T_k of the 64 elements in LDS
-        multi_spread_store<SYM_G>(a, m_lo, g, oi, i < a.N, ms, Ts[t & 63]);
+        if (i < a.N) term_epilogue_pre(a, i, e % R, y, ep);
     }
 }
 
@@ -2073,8 +2049,7 @@ static void launch_symm2(const MatvecArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((symm2_tiles_kernel<R>), dim3(npr * (npr + 1) / 2), dim3(256), 0, st, a.M,
                        reinterpret_cast<const double*>(a.x), reinterpret_cast<double*>(a.partial), a.ld, a.N,
                        a.x_stride, a.x_off);
-    if (a.accm) hipLaunchKernelGGL((sym2_reduce_kernel<R, true>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
-    else hipLaunchKernelGGL((sym2_reduce_kernel<R, false>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
+    hipLaunchKernelGGL((sym2_reduce_kernel<R>), dim3(nb, R), dim3(SYM_REDUCE_THREADS), 0, st, a);
 }
 
 // ------------------------------------------------------------------ symmetric, loads up front
@@ -2277,11 +2252,6 @@ void matvec_set_tuned(int N, int R, int tri, int variant) {
 }
 void matvec_force_variant(int v) { g_variant_override = v; }
 int matvec_variant_count() { return MV_VARIANTS; }
-
-// variants whose epilogue is term_epilogue or term_epilogue_pre (the split-K
-// GEMM and the symmetric half-storage kernels), which take a multi-step
-// series' accumulators
-bool matvec_variant_multi_ok(int variant) { return variant >= 15 && variant <= 21; }
 
 void launch_matvec_variant(const MatvecArgs& a, int variant, hipStream_t st) {
     if (variant == 21) {

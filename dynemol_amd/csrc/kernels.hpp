@@ -122,6 +122,28 @@ void launch_copy_matrix(const double* src, double* dst, long n, hipStream_t st);
 enum Tri { TRI_FULL = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
 enum Epi { EPI_PLAIN = 0, EPI_CHEB_FIRST = 1, EPI_CHEB = 2 };
 
+// Several time steps from one recurrence: accumulator q (q < m, layout of acc,
+// at accm + q * accm_stride) is sum_k coefm[q * coefm_stride + k] T_k over
+// k <= kq[q] (device table; kq[m - 1] is the series length).  The recurrence
+// writes T_k (k >= 1) to ring + ((k - 1) % PANEL) * ring_stride; a panel pass
+// adds terms k0..k1 (at most PANEL + 1 of them, k1 - k0 < PANEL after the
+// first) to every accumulator that takes any of them, in term order.  k0 = 0:
+// the pass starts the accumulators from zero and reads T_0 from phi.  n =
+// elements of rows < N (row-major, R per row).
+constexpr int MULTI_MAX = 10;  // accumulators of one multi-step series
+constexpr int PANEL = 8;       // terms per panel pass, and the ring length
+struct PanelArgs {
+    cplx* accm;
+    long accm_stride;
+    const cplx* coefm;
+    int coefm_stride;
+    const cplx* ring;
+    long ring_stride;
+    const cplx* phi;
+    const int* kq;
+    int m;
+    long n;
+};
 struct MatvecArgs {
     const double* M;  // column-major ld x ld; y_i = sum_j M[j + ld*i] x_j (= (M^T x)_i)
     int ld, N;
@@ -144,21 +166,29 @@ struct MatvecArgs {
     const float* x32;       // mixed mode: FP32 copy of x (same layout), read by the FP32 GEMM; may be null
     float* out32;           // mixed mode: the epilogue also writes T_k here in FP32 (same layout); may be null
     unsigned short* outb16; // precision mode 3: the epilogue also writes T_k here in bf16 (same layout); may be null
-    // Several time steps from one recurrence (EPI_CHEB_FIRST / EPI_CHEB): with
-    // accm set, accumulator q (layout of acc, at accm + q * accm_stride) takes
-    // coefm[q * coefm_stride + k] T_k for q in [m_lo, m_hi), in place of acc
-    // and coef; the kernels whose epilogue is term_epilogue or
-    // term_epilogue_pre support it (matvec_variant_multi_ok).
+    // A multi-step series' per-term accumulation (a null acc): with accm set,
+    // accumulator q in [m_lo, m_hi) (layout of acc, at accm + q * accm_stride)
+    // takes coefm[q * coefm_stride + k] T_k (EPI_CHEB), or is set to
+    // coefm[q][0] x + coefm[q][1] T_1 (EPI_CHEB_FIRST).  Only the symmetric
+    // block reduction (phase 2 of variants 16-18, 20, 21 and of the
+    // FP32-stored operator's term) honours it and reports so through
+    // take_multi_done; other terms ignore it, and their series add the terms
+    // to the accumulators in panel passes instead.
     cplx* accm;
     long accm_stride;
     const cplx* coefm;
     int coefm_stride;
     int m_lo, m_hi;
-    const int* kq;  // batch members: the accumulators' last terms (device), from which
-                    // the kernel takes m_lo = first q < m_hi - 1 with kq[q] >= k
 };
-constexpr int MULTI_MAX = 10;  // accumulators of one multi-step series
-bool matvec_variant_multi_ok(int variant);
+// true once after a term launch that accumulated a.accm itself
+bool take_multi_done();
+// A null acc leaves the accumulator out of the epilogue (out = T_k only): the
+// recurrence of a multi-step series, whose accumulators take the terms in
+// panels (launch_multi_panel).
+
+void launch_multi_panel(const PanelArgs& p, int k0, int k1, hipStream_t st);
+void launch_multi_panel_grouped(const PanelArgs* d_args, const int* task_desc, const int* task_blk, int ntask,
+                                int k0, int k1, hipStream_t st);
 int matvec_split_for(int N, int R);
 // complex elements of a.partial the symmetric two-phase variant needs at (ld, R)
 long sym_partial_cplx(int ld, int R);
@@ -192,7 +222,7 @@ int sym_term_segments(int ld, int* S);
 // reduce_groups: the phase-2 group count of the batch's systems (sym_reduce_groups_for
 // their ld; a batch whose systems differ in it uses the largest)
 void launch_sym_grouped(const MatvecArgs* d_args, const int* t1_desc, const int* t1_idx, int n1, const int* t2_desc,
-                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st, bool multi = false);
+                        const int* t2_idx, int n2, const int* sys_S, int R, int k, int reduce_groups, hipStream_t st);
 int sym_reduce_groups_for(int ld);
 
 void launch_zero_u32(unsigned* p, int n, hipStream_t st);
