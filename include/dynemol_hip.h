@@ -195,14 +195,13 @@ int dh_get_stats(dh_system *sys, double *out9);
  * context stream that runs them); no counterpart in the reference, whose
  * only timing hook is the GPU_TIMING wall-clock print (GPU_Interface.cpp:33-49). */
 int dh_set_timing(dh_system *sys, int on);
-/* Steps per Chebyshev recurrence in dh_propagate_steps (FP64 systems whose
- * autotuned term kernel is the split-K GEMM or a symmetric half-storage
- * kernel, the FP32-stored operator mode, and the FP32-product modes when the
- * right-hand sides R = 2 n_wp stay below N / 50, where the accumulators'
- * traffic stays small against the operator's): within a rebuild interval the
- * operator is fixed and the terms
- * T_k(Hh) phi do not depend on time, so one recurrence to the last step's K
- * collects every step's state, each with its own coefficients.  m = 0
+/* Steps per Chebyshev recurrence in dh_propagate_steps (every precision mode
+ * and term kernel): within a rebuild interval the operator is fixed and the
+ * terms T_k(Hh) phi do not depend on time, so one recurrence to the last
+ * step's K collects every step's state, each with its own coefficients (the
+ * symmetric half-storage terms add each term to the step accumulators in
+ * their epilogue; the other term kernels leave the terms in an 8-slot ring
+ * that a pass adds to the accumulators every 8 terms).  m = 0
  * (default): up to 10 steps per recurrence, within the interval and the
  * 4096-term coefficient table; m = 1: one recurrence per step, the same
  * kernels and bits as dh_propagate.  The steps' states agree to rounding
@@ -239,9 +238,10 @@ int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_te
  * own coefficients, as dh_propagate_steps does for one system): one norm
  * check, commit and output pass per recurrence, and the outputs read
  * afterwards are those of the final state.  The states agree with n_steps
- * calls of dh_batch_propagate to rounding.  Batches with members in other
- * precision modes, or whose autotuned term family is the dense one, step one
- * step at a time.  max_terms: the most terms of one recurrence.  No counterpart
+ * calls of dh_batch_propagate to rounding.  The grouped terms write each
+ * member's ring of 8 terms, and a grouped pass adds them to the members'
+ * accumulators every 8 terms.  Batches with members in other precision modes
+ * step one step at a time.  max_terms: the most terms of one recurrence.  No counterpart
  * in the reference. */
 int dh_batch_propagate_steps(dh_batch *batch, int n_steps, double dt, int *max_terms);
 /* Term kernel family the batch autotuner chose for this batch's wavepacket
