@@ -440,29 +440,53 @@ __global__ void zero_upper_blocks_kernel(double* A, int ld) {
     for (int i = threadIdx.x; i < top; i += blockDim.x) A[(long)i + (long)ld * j] = 0.0;
 }
 
+// Right-looking blocked Cholesky in panels of CHOL_PANEL 64-column blocks:
+// within a panel, block b is factored (potrf_diag_kernel), its column below is
+// solved against it, and the rank-64 update reaches only the panel's later
+// columns; the matrix right of the panel then takes one rank-256 update.  The
+// trailing matrix is read and written once per 256 columns instead of once per
+// 64 (at N = 4096 about 1.4 GB of update traffic instead of 5.7 GB), and its
+// GEMMs run with K = 256.
+constexpr int CHOL_PANEL = 4;
+
 void launch_cholesky(double* A, int ld, double* Dinv, int* info, hipStream_t st) {
     const int nb = ld / TILE;
-    for (int b = 0; b < nb; ++b) {
-        const int j0 = b * TILE;
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, ld, j0, Dinv, info);
-        const int rest = ld - j0 - TILE;
+    for (int p0 = 0; p0 < nb; p0 += CHOL_PANEL) {
+        const int pe = std::min(nb, p0 + CHOL_PANEL);  // blocks p0 .. pe-1
+        for (int b = p0; b < pe; ++b) {
+            const int j0 = b * TILE;
+            hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, st, A, ld, j0, Dinv, info);
+            const int rest = ld - j0 - TILE;
+            if (rest <= 0) continue;
+            // column: A21 <- A21 * L11^-T (Dinv holds L11^-1).  In place is
+            // safe: N = 64 gives one column tile per row tile, and a tile reads
+            // all 64 k-columns of its own rows before it writes them.
+            Gemm g{};
+            g.M = rest; g.N = TILE; g.K = TILE;
+            g.A = A + (j0 + TILE) + (long)ld * j0; g.lda = ld;
+            g.B = Dinv + (long)b * TILE * TILE; g.ldb = TILE; g.transB = 1;
+            g.C = A + (j0 + TILE) + (long)ld * j0; g.ldc = ld;
+            g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+            launch_gemm(g, st);
+            // the panel's later columns (lower tiles only): A22[:, :w] -= A21 A21[:w]^T
+            const int w = pe * TILE - (j0 + TILE);
+            if (w <= 0) continue;
+            Gemm u{};
+            u.M = rest; u.N = w; u.K = TILE;
+            u.A = A + (j0 + TILE) + (long)ld * j0; u.lda = ld;
+            u.B = A + (j0 + TILE) + (long)ld * j0; u.ldb = ld; u.transB = 1;
+            u.C = A + (j0 + TILE) + (long)ld * (j0 + TILE); u.ldc = ld;
+            u.alpha = -1.0; u.beta = 1.0; u.lower_only = 1; u.batch = 1;
+            launch_gemm(u, st);
+        }
+        // right of the panel (lower tiles only): A33 -= L31 L31^T, K = the panel's width
+        const int t0 = pe * TILE, rest = ld - t0;
         if (rest <= 0) continue;
-        // panel: A21 <- A21 * L11^-T (Dinv holds L11^-1).  In place is safe:
-        // N = 64 gives one column tile per row tile, and a tile reads all 64
-        // k-columns of its own rows before it writes them.
-        Gemm g{};
-        g.M = rest; g.N = TILE; g.K = TILE;
-        g.A = A + (j0 + TILE) + (long)ld * j0; g.lda = ld;
-        g.B = Dinv + (long)b * TILE * TILE; g.ldb = TILE; g.transB = 1;
-        g.C = A + (j0 + TILE) + (long)ld * j0; g.ldc = ld;
-        g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
-        launch_gemm(g, st);
-        // trailing update (lower tiles only): A22 -= A21 A21^T
         Gemm u{};
-        u.M = rest; u.N = rest; u.K = TILE;
-        u.A = A + (j0 + TILE) + (long)ld * j0; u.lda = ld;
-        u.B = A + (j0 + TILE) + (long)ld * j0; u.ldb = ld; u.transB = 1;
-        u.C = A + (j0 + TILE) + (long)ld * (j0 + TILE); u.ldc = ld;
+        u.M = rest; u.N = rest; u.K = (pe - p0) * TILE;
+        u.A = A + t0 + (long)ld * (p0 * TILE); u.lda = ld;
+        u.B = A + t0 + (long)ld * (p0 * TILE); u.ldb = ld; u.transB = 1;
+        u.C = A + t0 + (long)ld * t0; u.ldc = ld;
         u.alpha = -1.0; u.beta = 1.0; u.lower_only = 1; u.batch = 1;
         launch_gemm(u, st);
     }
