@@ -1128,8 +1128,13 @@ static int series_core(const SeriesOps& o, hipStream_t st) {
     cplx* prev = a.out;
     for (int k = 2; k <= o.K; ++k) {
         cplx* outb = panels ? slot(k) : (k == 2) ? o.x0 : const_cast<cplx*>(prev2);
-        if (panels)  // accumulators whose series has ended take no more terms
-            while (a.m_lo < o.m - 1 && o.Kq[a.m_lo] < k) ++a.m_lo;
+        // a term kernel that accumulates takes the terms two at a time: odd k
+        // adds T_{k-1} (its input) and T_k, even k nothing, except a last
+        // even k, which adds T_k alone; accumulators whose series has ended
+        // before the first term added take no more
+        a.m_pair = (k & 1) ? 2 : (k == o.K ? 1 : 0);
+        if (panels && a.m_pair > 0)
+            while (a.m_lo < o.m - 1 && o.Kq[a.m_lo] < k - (a.m_pair == 2 ? 1 : 0)) ++a.m_lo;
         a.epi = EPI_CHEB;
         a.x = prev;
         a.x2 = prev2;

@@ -1354,15 +1354,19 @@ constexpr int SYM_REDUCE_THREADS = SYM_G * 64;  // one 64-thread group per sourc
 static_assert(SYM_REDUCE_THREADS <= 1024, "phase-2 workgroup size");
 __host__ __device__ inline int sym_reduce_groups(int nb) { return nb > 32 ? 16 : 4; }
 
-// MULTI: a multi-step series' per-term accumulation (a.accm): T_k goes to
-// accumulators m_lo..m_hi-1 with their own coefficients.  Group g of the
-// workgroup owns accumulators m_lo + g + v G (v < QP), prefetched with their
-// coefficients alongside the epilogue's operands, and updates them with T_k
-// from LDS at the end, so that no thread updates more than QP of them.
+// MULTI: a multi-step series' accumulation (a.accm): the terms go to
+// accumulators m_lo..m_hi-1 with their own coefficients, two at a time
+// (a.m_pair = 2: c_{k-1} x + c_k T_k, where x = T_{k-1} is this term's input;
+// the term before, a.m_pair = 0, adds nothing; a.m_pair = 1: c_k T_k alone, a
+// series' last even term).  Group g of the workgroup owns accumulators m_lo +
+// g + v G (v < QP), prefetched with their coefficients alongside the
+// epilogue's operands, and updates them with x and T_k from LDS at the end, so
+// that no thread updates more than QP of them.  The additions are those of
+// one term at a time, in term order.
 template <int R, int G, bool MULTI = false>
 __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int chunk) {
     __shared__ cplx red[G * 64];
-    __shared__ cplx Ts[MULTI ? 64 : 1];
+    __shared__ cplx Ts[MULTI ? 64 : 1], Xs[MULTI ? 64 : 1];
     constexpr int QP = (MULTI_MAX + G - 1) / G;
     const int nb = a.ld / TILE;
     const int t = threadIdx.x, g = t >> 6, e = chunk * 64 + (t & 63);
@@ -1378,8 +1382,9 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
     // the epilogue's operands do not depend on the partials: fetch them first
     EpiOperands ep{};
     if (t < 64 && valid) ep = epilogue_operands(a, i, e % R);
-    const bool spread = MULTI && a.epi == EPI_CHEB;
-    cplx am[MULTI ? QP : 1], cm[MULTI ? QP : 1];
+    const bool spread = MULTI && a.epi == EPI_CHEB && a.m_pair > 0;
+    const bool pair = MULTI && a.m_pair == 2;
+    cplx am[MULTI ? QP : 1], cm[MULTI ? QP : 1], cx[MULTI ? QP : 1];
     if (spread) {
 #pragma unroll
         for (int v = 0; v < QP; ++v) {
@@ -1387,6 +1392,7 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
             if (valid && q < a.m_hi) {
                 am[v] = a.accm[(long)q * a.accm_stride + oi];
                 cm[v] = a.coefm[(long)q * a.coefm_stride + a.k];
+                if (pair) cx[v] = a.coefm[(long)q * a.coefm_stride + a.k - 1];
             }
         }
     }
@@ -1420,6 +1426,7 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
             const cplx T = term_epilogue_pre(a, i, e % R, y, ep);
             if (MULTI) {
                 Ts[t] = T;
+                Xs[t] = ep.xv;
                 if (a.epi == EPI_CHEB_FIRST)  // one term starts every accumulator
                     for (int q = a.m_lo; q < a.m_hi; ++q) {
                         const cplx* cq = a.coefm + (long)q * a.coefm_stride;
@@ -1430,14 +1437,21 @@ __device__ inline void sym_reduce_body(const MatvecArgs& a, int S, int B, int ch
         }
     }
     if (spread) {
-        __syncthreads();  // T_k of the 64 elements in LDS
+        __syncthreads();  // x and T_k of the 64 elements in LDS
         const cplx T = Ts[t & 63];
+        const cplx X = pair ? Xs[t & 63] : cplx{0.0, 0.0};
 #pragma unroll
         for (int v = 0; v < QP; ++v) {
             const int q = a.m_lo + g + v * G;
             if (valid && q < a.m_hi) {
+                cplx acc = am[v];
+                if (pair) {
+                    const cplx a1 = cmul(cx[v], X);
+                    acc.re += a1.re;
+                    acc.im += a1.im;
+                }
                 const cplx add = cmul(cm[v], T);
-                a.accm[(long)q * a.accm_stride + oi] = cplx{am[v].re + add.re, am[v].im + add.im};
+                a.accm[(long)q * a.accm_stride + oi] = cplx{acc.re + add.re, acc.im + add.im};
             }
         }
     }
@@ -1458,7 +1472,9 @@ bool take_multi_done() {
 
 template <int R>
 static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st) {
-    const bool multi = a.accm != nullptr;
+    // a term of a multi-step series that adds nothing (m_pair = 0) runs the
+    // plain kernel; the series is still accumulated by this kernel kind
+    const bool multi = a.accm != nullptr && (a.epi == EPI_CHEB_FIRST || a.m_pair > 0);
     if (sym_reduce_groups(nb) == 16) {
         if (multi) hipLaunchKernelGGL((sym_reduce_kernel<R, 16, true>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
         else hipLaunchKernelGGL((sym_reduce_kernel<R, 16, false>), dim3(nb, R), dim3(16 * 64), 0, st, a, S);
@@ -1466,7 +1482,7 @@ static void launch_sym_reduce(const MatvecArgs& a, int nb, int S, hipStream_t st
         if (multi) hipLaunchKernelGGL((sym_reduce_kernel<R, 4, true>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
         else hipLaunchKernelGGL((sym_reduce_kernel<R, 4, false>), dim3(nb, R), dim3(4 * 64), 0, st, a, S);
     }
-    if (multi) t_multi_done = true;
+    if (a.accm) t_multi_done = true;
 }
 
 // Batches: task i is (system task_desc[i], block * R + chunk = task_idx[i]),

@@ -179,6 +179,7 @@ struct MatvecArgs {
     const cplx* coefm;
     int coefm_stride;
     int m_lo, m_hi;
+    int m_pair;  // EPI_CHEB: 0 = adds nothing, 1 = c_k T_k, 2 = c_{k-1} x + c_k T_k (x = T_{k-1})
 };
 // true once after a term launch that accumulated a.accm itself
 bool take_multi_done();
