@@ -153,6 +153,41 @@ def _multi_vs_single(ctx, st, b, S, kets, geos, n_steps, every):
     B.close()
 
 
+# a step so short that a recurrence has fewer terms than one panel of 8: the
+# panel route (15) adds them all in the pass after the last term, the paired
+# route (18) ends on a single or a paired update depending on the last term's
+# parity; both must give the single steps' results
+@pytest.mark.parametrize("variant", [18, 15])
+def test_multi_step_short_series(ctx, variant):
+    st = ch_cluster(seed=1)
+    b = st.basis()
+    S = O.overlap(st.coords, b)
+    rng = np.random.default_rng(12)
+    kets = []
+    for _ in range(2):
+        x = rng.normal(size=b.N) + 1j * rng.normal(size=b.N)
+        kets.append(x / np.sqrt(np.real(np.conj(x) @ S @ x)))
+    dt, n_steps = 1.0e-8, 10
+    _say(f"ch N={b.N}: {n_steps} steps of {dt} ps, variant {variant}, short multi-step series vs one per step")
+    hip.force_matvec_variant(variant)
+    try:
+        A, B = _pair(ctx, st, kets, S)
+        B.set_steps_per_series(1)
+        pops_a, _, k_a = A.propagate_steps(n_steps, dt)
+        pops_b, _, _ = B.propagate_steps(n_steps, dt)
+        _say(f"  {k_a} terms in the last shared series")
+        assert k_a <= 8
+        assert np.abs(pops_a - pops_b).max() < 1e-13
+        for w in range(2):
+            _, ka = A.wavepacket(w)
+            _, kb = B.wavepacket(w)
+            assert np.abs(ka - kb).max() <= 1e-12 * np.abs(kb).max()
+        A.close()
+        B.close()
+    finally:
+        hip.force_matvec_variant(-1)
+
+
 def test_steps_argument_checks(ctx):
     st = ch_cluster(seed=1)
     s = hip.System(ctx, st.basis(), n_wp=1, n_frag=2)
