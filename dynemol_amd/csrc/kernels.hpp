@@ -131,7 +131,7 @@ enum Epi { EPI_PLAIN = 0, EPI_CHEB_FIRST = 1, EPI_CHEB = 2 };
 // the pass starts the accumulators from zero and reads T_0 from phi.  n =
 // elements of rows < N (row-major, R per row).
 constexpr int MULTI_MAX = 10;  // accumulators of one multi-step series
-constexpr int PANEL = 8;       // terms per panel pass, and the ring length
+constexpr int PANEL = 16;      // terms per panel pass, and the ring length
 struct PanelArgs {
     cplx* accm;
     long accm_stride;

@@ -200,8 +200,8 @@ int dh_set_timing(dh_system *sys, int on);
  * terms T_k(Hh) phi do not depend on time, so one recurrence to the last
  * step's K collects every step's state, each with its own coefficients (the
  * symmetric half-storage terms add each term to the step accumulators in
- * their epilogue; the other term kernels leave the terms in an 8-slot ring
- * that a pass adds to the accumulators every 8 terms).  m = 0
+ * their epilogue; the other term kernels leave the terms in a 16-slot ring
+ * that a pass adds to the accumulators every 16 terms).  m = 0
  * (default): up to 10 steps per recurrence, within the interval and the
  * 4096-term coefficient table; m = 1: one recurrence per step, the same
  * kernels and bits as dh_propagate.  The steps' states agree to rounding
@@ -239,8 +239,8 @@ int dh_batch_propagate(dh_batch *batch, double t_init, double t_max, int *max_te
  * check, commit and output pass per recurrence, and the outputs read
  * afterwards are those of the final state.  The states agree with n_steps
  * calls of dh_batch_propagate to rounding.  The grouped terms write each
- * member's ring of 8 terms, and a grouped pass adds them to the members'
- * accumulators every 8 terms.  Batches with members in other precision modes
+ * member's ring of 16 terms, and a grouped pass adds them to the members'
+ * accumulators every 16 terms.  Batches with members in other precision modes
  * step one step at a time.  max_terms: the most terms of one recurrence.  No counterpart
  * in the reference. */
 int dh_batch_propagate_steps(dh_batch *batch, int n_steps, double dt, int *max_terms);

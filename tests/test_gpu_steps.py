@@ -153,7 +153,7 @@ def _multi_vs_single(ctx, st, b, S, kets, geos, n_steps, every):
     B.close()
 
 
-# a step so short that a recurrence has fewer terms than one panel of 8: the
+# a step so short that a recurrence has fewer terms than one panel of 16: the
 # panel route (15) adds them all in the pass after the last term, the paired
 # route (18) ends on a single or a paired update depending on the last term's
 # parity; both must give the single steps' results
@@ -176,7 +176,7 @@ def test_multi_step_short_series(ctx, variant):
         pops_a, _, k_a = A.propagate_steps(n_steps, dt)
         pops_b, _, _ = B.propagate_steps(n_steps, dt)
         _say(f"  {k_a} terms in the last shared series")
-        assert k_a <= 8
+        assert k_a <= 16
         assert np.abs(pops_a - pops_b).max() < 1e-13
         for w in range(2):
             _, ka = A.wavepacket(w)

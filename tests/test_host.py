@@ -101,11 +101,11 @@ def test_mixed_footprint_counts_fp32_copies():
     """Serving admission counts the FP32 operator copies of a mixed-precision
     system, and in every mode the multi-step series' accumulators (10 of
     ld x 2 n_wp complex), their coefficient rows (10 x 4098 complex) and the
-    8-slot ring of terms (ld x 2 n_wp complex each)."""
+    16-slot ring of terms (ld x 2 n_wp complex each)."""
     from dynemol_amd import hip
     ld = 448
     base = int(hip.lib().dh_system_bytes(400, 2))
-    multi = (10 * (ld * 4 + 4098) + 8 * ld * 4) * 16
+    multi = (10 * (ld * 4 + 4098) + 16 * ld * 4) * 16
     assert hip.system_bytes(400, 2, hip.DH_PREC_MIXED) - base == 4 * ld * ld + 2 * ld * 4 * 2 * 4 + multi
     assert hip.system_bytes(400, 2, hip.DH_PREC_F32_STORAGE) - base == 4 * ld * ld + multi
     assert hip.system_bytes(400, 2) - base == multi
