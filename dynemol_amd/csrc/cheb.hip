@@ -2626,19 +2626,28 @@ int bessel_coef_host(double x, double phase, int kmax_alloc, double tol, cplx* c
 //   pop_f = Re sum_{i in f} conj(ket_i) bra_i  (data_output.f:242-263, DUAL_bra = conj(ket), DUAL_ket = bra)
 // and <H> = sum_i conj(phi_bra_i) (H_orth phi_ket)_i (ElHl_Chebyshev.f:329-371).
 // out: [bra (W) | AO_bra (W) | ket (W)] per row; phi: [bra (W) | ket (W)]; hk: N x W.
-__device__ void observables_body(int N, int W, int n_frag, const int* frag, const cplx* out, const cplx* phi,
-                                 const cplx* hk, double* pops, double* erg, double* norms, const int w) {
+// The fragment sums stay in registers: a row adds its population to every
+// fragment's sum, zero to all but its own (x + 0 = x), so no array is indexed
+// by the row's fragment (a dynamically indexed array would live in scratch
+// memory).  The loop is unrolled so that several rows' loads are in flight;
+// each thread's sums keep their row order.
+__device__ __forceinline__ void observables_body(int N, int W, int n_frag, const int* frag, const cplx* out,
+                                                 const cplx* phi, const cplx* hk, double* pops, double* erg,
+                                                 double* norms, const int w) {
     constexpr int MAXF = 16;
     __shared__ double sh[(MAXF + 5) * 4];
     double pf[MAXF + 1];
+#pragma unroll
     for (int f = 0; f <= MAXF; ++f) pf[f] = 0.0;
     double er = 0.0, ei = 0.0, nk = 0.0, nb = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const cplx b = out[(long)i * 3 * W + w];
         const cplx k = out[(long)i * 3 * W + 2 * W + w];
         const double p = k.re * b.re + k.im * b.im;  // Re(conj(k) b)
         const int f = frag[i];
-        if (f >= 0 && f < MAXF) pf[f] += p;
+#pragma unroll
+        for (int ff = 0; ff < MAXF; ++ff) pf[ff] += f == ff ? p : 0.0;
         pf[MAXF] += p;
         const cplx pb = phi[(long)i * 2 * W + w];
         const cplx pk = phi[(long)i * 2 * W + W + w];
@@ -2653,7 +2662,9 @@ __device__ void observables_body(int N, int W, int n_frag, const int* frag, cons
         v = wave_sum(v);
         if (lane == 0) sh[slot * 4 + wv] = v;
     };
-    for (int f = 0; f < n_frag && f < MAXF; ++f) red(pf[f], f);
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f)
+        if (f < n_frag) red(pf[f], f);
     red(pf[MAXF], MAXF);
     red(er, MAXF + 1);
     red(ei, MAXF + 2);
@@ -2687,6 +2698,7 @@ __global__ __launch_bounds__(256) void column_norms_kernel(int N, int R, const c
     __shared__ double sh[4];
     const int c = blockIdx.x;
     double s = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const cplx v = x[(long)i * R + c];
         s += v.re * v.re + v.im * v.im;
@@ -2701,6 +2713,7 @@ __global__ __launch_bounds__(256) void grouped_norms_kernel(const NormTask* task
     __shared__ double sh[4];
     const NormTask t = tasks[blockIdx.x];
     double s = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < t.N; i += blockDim.x) {
         const cplx v = t.x[(long)i * t.R + t.c];
         s += v.re * v.re + v.im * v.im;
