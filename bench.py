@@ -636,6 +636,11 @@ def main():
     traffic = MEASURED_TRAFFIC_BYTES.get(variant) if N == 4096 else None
     value = 2 * args.steps / g["elapsed"]
     prop_only = g["prop_only"]
+    # the same term alone (HIP events around repeated launches on a zero-filled
+    # operator of the same shape, one accumulator, nothing beside it): the
+    # figure a kernel trace's phase 1 + phase 2 averages compare with
+    from dynemol_amd import hip
+    alone_us = hip.time_matvec(N, R, variant, reps=50)
     out = {
         "metric": "wavepacket-steps/s at N=4096 (mixed vs FP64, pop.err<=1e-6) + serve p50/p99",
         "value": round(value, 3),
@@ -665,7 +670,9 @@ def main():
                      # the timed region's rebuilds run on a second stream beside the terms (dh_propagate_steps),
                      # so this average includes their interference
                      "dense_equivalent_bytes": term_bytes(N, R),
-                     "dense_equivalent_frac": round(term_bytes(N, R) / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4)},
+                     "dense_equivalent_frac": round(term_bytes(N, R) / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "alone_us": round(alone_us, 3),
+                     "alone_frac": round(bytes_term / (alone_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
         "propagation_only_value": round(prop_only, 3),
         # terms launched in the timed region per step (an interval's steps share recurrences)
         "chebyshev_terms_per_step": round(g["series_launches"] / args.steps, 1),
